@@ -1,0 +1,210 @@
+#include "execution.hpp"
+
+#include <utility>
+
+#include "hip_check.hpp"
+#include "spfft/exceptions.hpp"
+
+namespace spfft {
+
+namespace {
+
+// Host or device view of a user pointer (reference:
+// src/gpu_util/gpu_pointer_translation.hpp:41-71). Device and managed
+// allocations are used in place; anything else goes through a copy.
+template <typename P>
+std::pair<P*, P*> translate_pointer(P* ptr) {
+  hipPointerAttribute_t attr{};
+  const hipError_t status = hipPointerGetAttributes(&attr, static_cast<const void*>(ptr));
+  if (status != hipSuccess) {
+    (void)hipGetLastError();
+    if (status != hipErrorInvalidValue) check_hip(status);
+    return {ptr, nullptr};
+  }
+  if (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged) {
+    return {nullptr, static_cast<P*>(attr.devicePointer)};
+  }
+  return {attr.hostPointer ? static_cast<P*>(attr.hostPointer) : ptr, nullptr};
+}
+
+}  // namespace
+
+template <typename T>
+ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
+                              const std::shared_ptr<GridInternal<T>>& grid)
+    : map_(map),
+      grid_(grid),
+      scale_(static_cast<T>(1.0 / static_cast<double>(static_cast<unsigned long long>(map->dimX) *
+                                                      static_cast<unsigned long long>(map->dimY) *
+                                                      static_cast<unsigned long long>(map->dimZ)))) {
+  check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  check_hip(hipEventCreateWithFlags(&startEvent_, hipEventDisableTiming));
+  check_hip(hipEventCreateWithFlags(&endEvent_, hipEventDisableTiming));
+
+  const StickTables t = build_stick_tables(*map_);
+  numCols_ = static_cast<int>(t.colX.size());
+  entryValue_ = DeviceBuffer<int>::from_host(t.entryValue);
+  entrySlot_ = DeviceBuffer<int>::from_host(t.entrySlot);
+  stickBegin_ = DeviceBuffer<int>::from_host(t.stickBegin);
+  stickY_ = DeviceBuffer<int>::from_host(t.stickY);
+  stickCol_ = DeviceBuffer<int>::from_host(t.stickCol);
+  colX_ = DeviceBuffer<int>::from_host(t.colX);
+  colBegin_ = DeviceBuffer<int>::from_host(t.colBegin);
+
+  fftX_ = make_fft_desc(map_->dimX);
+  fftY_ = make_fft_desc(map_->dimY);
+  fftZ_ = make_fft_desc(map_->dimZ);
+  size_t lds = 0;
+  zBatch_ = stage_batch<T>(map_->dimZ, std::max(1, map_->num_sticks()), &lds);
+  yBatch_ = stage_batch<T>(map_->dimY, std::max(1, numCols_), &lds);
+  xBatch_ = stage_batch<T>(map_->dimX, map_->dimY, &lds);
+
+  twX_ = DeviceBuffer<Cx<T>>(map_->dimX);
+  twY_ = DeviceBuffer<Cx<T>>(map_->dimY);
+  twZ_ = DeviceBuffer<Cx<T>>(map_->dimZ);
+  generate_twiddles<T>(twX_.data(), map_->dimX, stream_);
+  generate_twiddles<T>(twY_.data(), map_->dimY, stream_);
+  generate_twiddles<T>(twZ_.data(), map_->dimZ, stream_);
+  check_hip(hipStreamSynchronize(stream_));
+}
+
+template <typename T>
+ExecutionGPU<T>::~ExecutionGPU() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (startEvent_) (void)hipEventDestroy(startEvent_);
+  if (endEvent_) (void)hipEventDestroy(endEvent_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+template <typename T>
+size_t ExecutionGPU<T>::space_domain_size() const {
+  const size_t n = static_cast<size_t>(map_->dimX) * map_->dimY * map_->dimZ;
+  return map_->type == SPFFT_TRANS_R2C ? n : 2 * n;
+}
+
+template <typename T>
+void ExecutionGPU<T>::begin() {
+  // An error left by earlier GPU work is reported, not absorbed
+  // (reference execution_gpu.cpp:256-258).
+  if (hipGetLastError() != hipSuccess) throw GPUPrecedingError();
+  check_hip(hipEventRecord(startEvent_, nullptr));
+  check_hip(hipStreamWaitEvent(stream_, startEvent_, 0));
+}
+
+template <typename T>
+StageArgs<T> ExecutionGPU<T>::base_args() const {
+  StageArgs<T> a;
+  a.nx = map_->dimX;
+  a.ny = map_->dimY;
+  a.nz = map_->dimZ;
+  a.nxf = map_->dimXFreq;
+  a.nst = map_->num_sticks();
+  a.ncol = numCols_;
+  a.r2c = map_->type == SPFFT_TRANS_R2C ? 1 : 0;
+  a.zeroZeroStick = map_->zeroZeroStick;
+  a.entryValue = entryValue_.data();
+  a.entrySlot = entrySlot_.data();
+  a.stickBegin = stickBegin_.data();
+  a.stickY = stickY_.data();
+  a.stickCol = stickCol_.data();
+  a.colX = colX_.data();
+  a.colBegin = colBegin_.data();
+  a.zbuf = reinterpret_cast<Cx<T>*>(grid_->sticks_gpu());
+  a.ybuf = reinterpret_cast<Cx<T>*>(grid_->work_gpu());
+  a.scale = scale_;
+  return a;
+}
+
+template <typename T>
+void ExecutionGPU<T>::backward(const T* input, T* output) {
+  begin();
+  const size_t numValueReals = 2 * static_cast<size_t>(map_->num_values());
+  auto in = translate_pointer(input);
+  auto out = translate_pointer(output);
+  const T* inGPU = in.second;
+  if (numValueReals > 0 && !inGPU) {
+    // Host input: stage in the work buffer, which is free until the y stage.
+    check_hip(hipMemcpyAsync(grid_->work_gpu(), in.first, numValueReals * sizeof(T),
+                             hipMemcpyHostToDevice, stream_));
+    inGPU = grid_->work_gpu();
+  }
+  T* outGPU = out.second ? out.second : grid_->space_gpu();
+
+  StageArgs<T> a = base_args();
+  a.in = inGPU;
+  a.out = outGPU;
+  if (a.nst > 0) {
+    a.fft = fftZ_;
+    a.tw = twZ_.data();
+    a.batch = zBatch_;
+    launch_z_backward<T>(a, stream_);
+    a.fft = fftY_;
+    a.tw = twY_.data();
+    a.batch = yBatch_;
+    launch_y_backward<T>(a, stream_);
+  }
+  a.fft = fftX_;
+  a.tw = twX_.data();
+  a.batch = xBatch_;
+  launch_x_backward<T>(a, stream_);
+
+  if (out.first) {
+    check_hip(hipMemcpyAsync(out.first, outGPU, space_domain_size() * sizeof(T),
+                             hipMemcpyDeviceToHost, stream_));
+  }
+}
+
+template <typename T>
+void ExecutionGPU<T>::forward(const T* input, T* output, SpfftScalingType scaling) {
+  begin();
+  const size_t numValueReals = 2 * static_cast<size_t>(map_->num_values());
+  auto in = translate_pointer(input);
+  auto out = translate_pointer(output);
+  const T* inGPU = in.second;
+  if (!inGPU) {
+    check_hip(hipMemcpyAsync(grid_->space_gpu(), in.first, space_domain_size() * sizeof(T),
+                             hipMemcpyHostToDevice, stream_));
+    inGPU = grid_->space_gpu();
+  }
+  if (numValueReals == 0) return;
+  // Host output: the z stage writes into the work buffer, which is free again
+  // once the y stage has consumed it.
+  T* outGPU = out.second ? out.second : grid_->work_gpu();
+
+  StageArgs<T> a = base_args();
+  a.in = inGPU;
+  a.out = outGPU;
+  a.useScale = scaling == SPFFT_FULL_SCALING ? 1 : 0;
+  a.fft = fftX_;
+  a.tw = twX_.data();
+  a.batch = xBatch_;
+  launch_x_forward<T>(a, stream_);
+  a.fft = fftY_;
+  a.tw = twY_.data();
+  a.batch = yBatch_;
+  launch_y_forward<T>(a, stream_);
+  a.fft = fftZ_;
+  a.tw = twZ_.data();
+  a.batch = zBatch_;
+  launch_z_forward<T>(a, stream_);
+
+  if (out.first) {
+    check_hip(hipMemcpyAsync(out.first, outGPU, numValueReals * sizeof(T), hipMemcpyDeviceToHost,
+                             stream_));
+  }
+}
+
+template <typename T>
+void ExecutionGPU<T>::synchronize(SpfftExecType mode) {
+  if (mode == SPFFT_EXEC_ASYNCHRONOUS) {
+    check_hip(hipEventRecord(endEvent_, stream_));
+    check_hip(hipStreamWaitEvent(nullptr, endEvent_, 0));
+  } else {
+    check_hip(hipStreamSynchronize(stream_));
+  }
+}
+
+template class ExecutionGPU<double>;
+template class ExecutionGPU<float>;
+
+}  // namespace spfft

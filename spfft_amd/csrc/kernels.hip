@@ -1,0 +1,539 @@
+// HIP kernels of the sparse 3D FFT for CDNA4 (gfx950), generic-length path.
+//
+// Every stage kernel stages a batch of 1D lines in LDS, runs a mixed-radix
+// Stockham FFT over them (radix 2/3/4/5/7/8 butterflies in registers, any
+// other prime as a direct DFT from a twiddle table) and writes the result
+// with the next stage's layout, so the transpose between stages happens at
+// the LDS -> HBM boundary. Twiddles come from one table per length holding
+// exp(-2 pi i t / n) generated on the GPU in FP64; the backward direction uses
+// the conjugate, which is exact.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "hip_check.hpp"
+#include "kernels.hpp"
+
+namespace spfft {
+
+extern __shared__ __attribute__((aligned(16))) unsigned char spfft_lds[];
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr size_t kLdsDefault = 64 * 1024;
+constexpr size_t kLdsMax = 160 * 1024;
+
+template <typename T>
+__device__ __forceinline__ Cx<T> cadd(Cx<T> a, Cx<T> b) {
+  return {a.re + b.re, a.im + b.im};
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> csub(Cx<T> a, Cx<T> b) {
+  return {a.re - b.re, a.im - b.im};
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> cmul(Cx<T> a, Cx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> cscale(Cx<T> a, T s) {
+  return {a.re * s, a.im * s};
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> cconj(Cx<T> a) {
+  return {a.re, -a.im};
+}
+template <typename T>
+__device__ __forceinline__ bool cnonzero(Cx<T> a) {
+  return a.re != T(0) || a.im != T(0);
+}
+// Multiply by -i (forward) or +i (backward).
+template <bool BWD, typename T>
+__device__ __forceinline__ Cx<T> rot(Cx<T> a) {
+  return BWD ? Cx<T>{-a.im, a.re} : Cx<T>{a.im, -a.re};
+}
+template <bool BWD, typename T>
+__device__ __forceinline__ Cx<T> twiddle(const Cx<T>* __restrict__ w, int t) {
+  const Cx<T> v = w[t];
+  return BWD ? cconj(v) : v;
+}
+
+// ---- butterflies: in-register DFT of length R with sign -1 (fwd) / +1 (bwd)
+
+template <typename T, int R, bool BWD>
+struct Butterfly {
+  // Direct DFT from the length-n table: W_R^m = W_n^(m n / R).
+  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>* __restrict__ w, int n) {
+    Cx<T> out[R];
+    const int step = n / R;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      Cx<T> acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) acc = cadd(acc, cmul(v[r], twiddle<BWD>(w, ((r * q) % R) * step)));
+      out[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = out[q];
+  }
+};
+
+template <typename T, bool BWD>
+struct Butterfly<T, 2, BWD> {
+  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
+    const Cx<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <typename T, bool BWD>
+struct Butterfly<T, 4, BWD> {
+  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
+    const Cx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const Cx<T> t2 = cadd(v[1], v[3]), t3 = rot<BWD>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+template <typename T, bool BWD>
+struct Butterfly<T, 8, BWD> {
+  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>* w, int n) {
+    Cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    Cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    Butterfly<T, 4, BWD>::run(e, w, n);
+    Butterfly<T, 4, BWD>::run(o, w, n);
+    const T h = T(0.70710678118654752440);
+    // o[k] *= exp(-/+ 2 pi i k / 8)
+    const Cx<T> o1 = BWD ? Cx<T>{h * (o[1].re - o[1].im), h * (o[1].re + o[1].im)}
+                         : Cx<T>{h * (o[1].re + o[1].im), h * (o[1].im - o[1].re)};
+    const Cx<T> o2 = rot<BWD>(o[2]);
+    const Cx<T> o3 = BWD ? Cx<T>{-h * (o[3].re + o[3].im), h * (o[3].re - o[3].im)}
+                         : Cx<T>{h * (o[3].im - o[3].re), -h * (o[3].re + o[3].im)};
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+  }
+};
+
+template <typename T, bool BWD>
+struct Butterfly<T, 3, BWD> {
+  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
+    const T s3 = T(0.86602540378443864676);  // sin(2 pi / 3)
+    const Cx<T> t1 = cadd(v[1], v[2]);
+    const Cx<T> t2 = csub(v[1], v[2]);
+    const Cx<T> m = csub(v[0], cscale(t1, T(0.5)));
+    const Cx<T> r = rot<BWD>(cscale(t2, s3));
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  }
+};
+
+template <typename T, bool BWD>
+struct Butterfly<T, 5, BWD> {
+  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
+    const T c1 = T(0.30901699437494742410);   // cos(2 pi / 5)
+    const T c2 = T(-0.80901699437494742410);  // cos(4 pi / 5)
+    const T s1 = T(0.95105651629515357212);   // sin(2 pi / 5)
+    const T s2 = T(0.58778525229247312917);   // sin(4 pi / 5)
+    const Cx<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const Cx<T> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const Cx<T> b1 = cadd(v[0], cadd(cscale(t1, c1), cscale(t2, c2)));
+    const Cx<T> b2 = cadd(v[0], cadd(cscale(t1, c2), cscale(t2, c1)));
+    const Cx<T> d1 = rot<BWD>(cadd(cscale(t3, s1), cscale(t4, s2)));
+    const Cx<T> d2 = rot<BWD>(csub(cscale(t3, s2), cscale(t4, s1)));
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(b1, d1);
+    v[4] = csub(b1, d1);
+    v[2] = cadd(b2, d2);
+    v[3] = csub(b2, d2);
+  }
+};
+
+// ---- one Stockham pass over `batch` contiguous lines of length n in LDS.
+// Ns = product of the radices already applied. Butterfly j reads
+// src[j + r n/R], twiddles by W^(k r n/(Ns R)) with k = j mod Ns and writes
+// dst[(j / Ns) Ns R + k + r Ns]; after the last pass the order is natural.
+template <typename T, int R, bool BWD>
+__device__ __forceinline__ void stockham_pass(const Cx<T>* __restrict__ src,
+                                              Cx<T>* __restrict__ dst, int n, int Ns, int batch,
+                                              const Cx<T>* __restrict__ w) {
+  const int m = n / R;
+  const int twStep = n / (Ns * R);
+  const int total = batch * m;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int b = t / m;
+    const int j = t - b * m;
+    const Cx<T>* s = src + b * n;
+    Cx<T>* d = dst + b * n;
+    const int k = j % Ns;
+    Cx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = s[j + r * m];
+    if (k != 0) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<BWD>(w, k * r * twStep));
+    }
+    Butterfly<T, R, BWD>::run(v, w, n);
+    const int o = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[o + r * Ns] = v[r];
+  }
+}
+
+// Any radix (primes above 7): one thread per output, direct DFT.
+template <typename T, bool BWD>
+__device__ __forceinline__ void stockham_pass_any(const Cx<T>* __restrict__ src,
+                                                  Cx<T>* __restrict__ dst, int n, int R, int Ns,
+                                                  int batch, const Cx<T>* __restrict__ w) {
+  const int m = n / R;
+  const int twStep = n / (Ns * R);
+  const int total = batch * n;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int b = t / n;
+    const int rem = t - b * n;
+    const int q = rem / m;
+    const int j = rem - q * m;
+    const Cx<T>* s = src + b * n;
+    const int k = j % Ns;
+    Cx<T> acc = {T(0), T(0)};
+    for (int r = 0; r < R; ++r) {
+      int e = k * r * twStep + ((r * q) % R) * m;
+      if (e >= n) e -= n;
+      acc = cadd(acc, cmul(s[j + r * m], twiddle<BWD>(w, e)));
+    }
+    dst[b * n + (j / Ns) * Ns * R + k + q * Ns] = acc;
+  }
+}
+
+// Full FFT of `batch` lines staged in a (scratch b); returns the buffer that
+// holds the result. Caller synchronises before the call.
+template <typename T, bool BWD>
+__device__ Cx<T>* lds_fft(Cx<T>* a, Cx<T>* b, const FftDesc& f, int batch,
+                          const Cx<T>* __restrict__ w) {
+  int Ns = 1;
+  for (int p = 0; p < f.npass; ++p) {
+    const int R = f.radix[p];
+    switch (R) {
+      case 2: stockham_pass<T, 2, BWD>(a, b, f.n, Ns, batch, w); break;
+      case 3: stockham_pass<T, 3, BWD>(a, b, f.n, Ns, batch, w); break;
+      case 4: stockham_pass<T, 4, BWD>(a, b, f.n, Ns, batch, w); break;
+      case 5: stockham_pass<T, 5, BWD>(a, b, f.n, Ns, batch, w); break;
+      case 7: stockham_pass<T, 7, BWD>(a, b, f.n, Ns, batch, w); break;
+      case 8: stockham_pass<T, 8, BWD>(a, b, f.n, Ns, batch, w); break;
+      default: stockham_pass_any<T, BWD>(a, b, f.n, R, Ns, batch, w); break;
+    }
+    __syncthreads();
+    Cx<T>* t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+// Hermitian fill of one line held in LDS, with the reference's rule
+// (src/symmetry/symmetry_host.hpp:47-90): for each pair (k, n-k), a non-zero
+// lower entry overwrites the upper one with its conjugate, otherwise a
+// non-zero upper entry fills the lower one; a non-zero self-paired entry
+// (k = n/2) is conjugated. Exact compares, as in the reference.
+template <typename T>
+__device__ __forceinline__ void hermitian_fill_line(Cx<T>* line, int n) {
+  for (int k = 1 + threadIdx.x; 2 * k <= n; k += blockDim.x) {
+    const int kk = n - k;
+    if (kk == k) {
+      if (cnonzero(line[k])) line[k] = cconj(line[k]);
+    } else if (cnonzero(line[k])) {
+      line[kk] = cconj(line[k]);
+    } else if (cnonzero(line[kk])) {
+      line[k] = cconj(line[kk]);
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void zero_lds(Cx<T>* a, int count) {
+  for (int t = threadIdx.x; t < count; t += blockDim.x) a[t] = Cx<T>{T(0), T(0)};
+}
+
+// ---------------------------------------------------------------- backward
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_z_backward(StageArgs<T> a) {
+  const int nz = a.nz;
+  Cx<T>* A = reinterpret_cast<Cx<T>*>(spfft_lds);
+  Cx<T>* B = A + a.batch * nz;
+  const int s0 = blockIdx.x * a.batch;
+  const int ns = min(a.batch, a.nst - s0);
+  zero_lds(A, ns * nz);
+  __syncthreads();
+  const int e0 = a.stickBegin[s0], e1 = a.stickBegin[s0 + ns];
+  const int base = s0 * nz;
+  for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int slot = a.entrySlot[e];
+    if (e + 1 < e1 && a.entrySlot[e + 1] == slot) continue;  // repeated triplet: last one wins
+    const size_t v = static_cast<size_t>(a.entryValue[e]);
+    A[slot - base] = Cx<T>{a.in[2 * v], a.in[2 * v + 1]};
+  }
+  __syncthreads();
+  if (a.r2c && s0 == 0 && a.zeroZeroStick == 0) {
+    hermitian_fill_line(A, nz);
+    __syncthreads();
+  }
+  const Cx<T>* r = lds_fft<T, true>(A, B, a.fft, ns, a.tw);
+  for (int t = threadIdx.x; t < ns * nz; t += blockDim.x) {
+    const int sl = t % ns, z = t / ns;
+    a.zbuf[static_cast<size_t>(z) * a.nst + s0 + sl] = r[sl * nz + z];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_y_backward(StageArgs<T> a) {
+  const int ny = a.ny;
+  Cx<T>* A = reinterpret_cast<Cx<T>*>(spfft_lds);
+  Cx<T>* B = A + a.batch * ny;
+  const int z = blockIdx.y;
+  const int u0 = blockIdx.x * a.batch;
+  const int nc = min(a.batch, a.ncol - u0);
+  zero_lds(A, nc * ny);
+  __syncthreads();
+  const Cx<T>* row = a.zbuf + static_cast<size_t>(z) * a.nst;
+  for (int s = a.colBegin[u0] + threadIdx.x; s < a.colBegin[u0 + nc]; s += blockDim.x) {
+    A[(a.stickCol[s] - u0) * ny + a.stickY[s]] = row[s];
+  }
+  __syncthreads();
+  if (a.r2c && u0 == 0 && a.colX[0] == 0) {
+    hermitian_fill_line(A, ny);
+    __syncthreads();
+  }
+  const Cx<T>* r = lds_fft<T, true>(A, B, a.fft, nc, a.tw);
+  for (int t = threadIdx.x; t < nc * ny; t += blockDim.x) {
+    const int ul = t % nc, y = t / nc;
+    a.ybuf[(static_cast<size_t>(z) * ny + y) * a.ncol + u0 + ul] = r[ul * ny + y];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_x_backward(StageArgs<T> a) {
+  const int nx = a.nx, ny = a.ny, ncol = a.ncol;
+  Cx<T>* A = reinterpret_cast<Cx<T>*>(spfft_lds);
+  Cx<T>* B = A + a.batch * nx;
+  const int z = blockIdx.y;
+  const int y0 = blockIdx.x * a.batch;
+  const int nr = min(a.batch, ny - y0);
+  zero_lds(A, nr * nx);
+  __syncthreads();
+  const Cx<T>* src = a.ybuf + (static_cast<size_t>(z) * ny + y0) * ncol;
+  for (int t = threadIdx.x; t < nr * ncol; t += blockDim.x) {
+    const int yl = t / ncol, u = t - yl * ncol;
+    A[yl * nx + a.colX[u]] = src[t];
+  }
+  __syncthreads();
+  if (a.r2c) {
+    // Complete the hermitian half along x; the imaginary parts of the DC and
+    // (even length) Nyquist entries do not contribute to a real signal.
+    for (int t = threadIdx.x; t < nr * nx; t += blockDim.x) {
+      const int yl = t / nx, k = t - yl * nx;
+      Cx<T>* line = A + yl * nx;
+      if (k >= a.nxf) {
+        line[k] = cconj(line[nx - k]);
+      } else if (k == 0 || 2 * k == nx) {
+        line[k].im = T(0);
+      }
+    }
+    __syncthreads();
+  }
+  const Cx<T>* r = lds_fft<T, true>(A, B, a.fft, nr, a.tw);
+  const size_t outBase = (static_cast<size_t>(z) * ny + y0) * nx;
+  if (a.r2c) {
+    for (int t = threadIdx.x; t < nr * nx; t += blockDim.x) a.out[outBase + t] = r[t].re;
+  } else {
+    Cx<T>* out = reinterpret_cast<Cx<T>*>(a.out);
+    for (int t = threadIdx.x; t < nr * nx; t += blockDim.x) out[outBase + t] = r[t];
+  }
+}
+
+// ----------------------------------------------------------------- forward
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_x_forward(StageArgs<T> a) {
+  const int nx = a.nx, ny = a.ny, ncol = a.ncol;
+  Cx<T>* A = reinterpret_cast<Cx<T>*>(spfft_lds);
+  Cx<T>* B = A + a.batch * nx;
+  const int z = blockIdx.y;
+  const int y0 = blockIdx.x * a.batch;
+  const int nr = min(a.batch, ny - y0);
+  const size_t inBase = (static_cast<size_t>(z) * ny + y0) * nx;
+  if (a.r2c) {
+    for (int t = threadIdx.x; t < nr * nx; t += blockDim.x) A[t] = Cx<T>{a.in[inBase + t], T(0)};
+  } else {
+    const Cx<T>* in = reinterpret_cast<const Cx<T>*>(a.in);
+    for (int t = threadIdx.x; t < nr * nx; t += blockDim.x) A[t] = in[inBase + t];
+  }
+  __syncthreads();
+  const Cx<T>* r = lds_fft<T, false>(A, B, a.fft, nr, a.tw);
+  Cx<T>* dst = a.ybuf + (static_cast<size_t>(z) * ny + y0) * ncol;
+  for (int t = threadIdx.x; t < nr * ncol; t += blockDim.x) {
+    const int yl = t / ncol, u = t - yl * ncol;
+    dst[t] = r[yl * nx + a.colX[u]];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_y_forward(StageArgs<T> a) {
+  const int ny = a.ny;
+  Cx<T>* A = reinterpret_cast<Cx<T>*>(spfft_lds);
+  Cx<T>* B = A + a.batch * ny;
+  const int z = blockIdx.y;
+  const int u0 = blockIdx.x * a.batch;
+  const int nc = min(a.batch, a.ncol - u0);
+  for (int t = threadIdx.x; t < nc * ny; t += blockDim.x) {
+    const int ul = t % nc, y = t / nc;
+    A[ul * ny + y] = a.ybuf[(static_cast<size_t>(z) * ny + y) * a.ncol + u0 + ul];
+  }
+  __syncthreads();
+  const Cx<T>* r = lds_fft<T, false>(A, B, a.fft, nc, a.tw);
+  Cx<T>* row = a.zbuf + static_cast<size_t>(z) * a.nst;
+  for (int s = a.colBegin[u0] + threadIdx.x; s < a.colBegin[u0 + nc]; s += blockDim.x) {
+    row[s] = r[(a.stickCol[s] - u0) * ny + a.stickY[s]];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_z_forward(StageArgs<T> a) {
+  const int nz = a.nz;
+  Cx<T>* A = reinterpret_cast<Cx<T>*>(spfft_lds);
+  Cx<T>* B = A + a.batch * nz;
+  const int s0 = blockIdx.x * a.batch;
+  const int ns = min(a.batch, a.nst - s0);
+  for (int t = threadIdx.x; t < ns * nz; t += blockDim.x) {
+    const int sl = t % ns, z = t / ns;
+    A[sl * nz + z] = a.zbuf[static_cast<size_t>(z) * a.nst + s0 + sl];
+  }
+  __syncthreads();
+  const Cx<T>* r = lds_fft<T, false>(A, B, a.fft, ns, a.tw);
+  const int base = s0 * nz;
+  for (int e = a.stickBegin[s0] + threadIdx.x; e < a.stickBegin[s0 + ns]; e += blockDim.x) {
+    Cx<T> c = r[a.entrySlot[e] - base];
+    if (a.useScale) c = cscale(c, a.scale);
+    const size_t v = static_cast<size_t>(a.entryValue[e]);
+    a.out[2 * v] = c.re;
+    a.out[2 * v + 1] = c.im;
+  }
+}
+
+template <typename T>
+__global__ void k_twiddles(Cx<T>* tw, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  // Angle reduced to (-pi, pi] so that the argument of sincospi is exact up
+  // to one rounding of t / n.
+  const int tt = (2 * t > n) ? t - n : t;
+  double s, c;
+  sincospi(-2.0 * static_cast<double>(tt) / static_cast<double>(n), &s, &c);
+  tw[t] = Cx<T>{static_cast<T>(c), static_cast<T>(s)};
+}
+
+template <typename Kernel>
+void launch_stage(Kernel kernel, dim3 grid, size_t lds, hipStream_t stream,
+                  const void* argsPtr) {
+  if (lds > kLdsDefault) {
+    check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(lds)));
+  }
+  (void)argsPtr;
+}
+
+}  // namespace
+
+FftDesc make_fft_desc(int n) {
+  FftDesc d;
+  d.n = n;
+  int m = n;
+  auto push = [&](int r) {
+    if (d.npass >= kMaxPasses) throw GPUFFTError();
+    d.radix[d.npass++] = r;
+    m /= r;
+  };
+  while (m % 8 == 0) push(8);
+  while (m % 4 == 0) push(4);
+  while (m % 2 == 0) push(2);
+  for (int r : {3, 5, 7}) {
+    while (m % r == 0) push(r);
+  }
+  for (int p = 11; m > 1; p += 2) {
+    if (static_cast<long long>(p) * p > m) p = m;
+    while (m % p == 0) push(p);
+  }
+  return d;
+}
+
+template <typename T>
+int stage_batch(int n, int count, size_t* ldsBytes) {
+  const size_t perLine = 2 * static_cast<size_t>(n) * sizeof(Cx<T>);
+  if (perLine > kLdsMax) throw GPUFFTError();
+  int batch = static_cast<int>(std::max<size_t>(1, kLdsDefault / perLine));
+  batch = std::min(batch, 64);
+  batch = std::max(1, std::min(batch, count));
+  *ldsBytes = perLine * batch;
+  return batch;
+}
+
+template <typename T>
+void generate_twiddles(Cx<T>* tw, int n, hipStream_t stream) {
+  const int blocks = (n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_twiddles<T>, dim3(blocks), dim3(kBlock), 0, stream, tw, n);
+  check_launch();
+}
+
+#define SPFFT_STAGE_LAUNCHER(NAME, KERNEL, LEN, COUNT, GRID)                      \
+  template <typename T>                                                          \
+  void NAME(const StageArgs<T>& a, hipStream_t stream) {                         \
+    size_t lds = 0;                                                              \
+    (void)stage_batch<T>(a.LEN, COUNT, &lds);                                    \
+    lds = 2 * static_cast<size_t>(a.LEN) * sizeof(Cx<T>) * a.batch;              \
+    launch_stage(KERNEL<T>, GRID, lds, stream, &a);                              \
+    hipLaunchKernelGGL(KERNEL<T>, GRID, dim3(kBlock), lds, stream, a);           \
+    check_launch();                                                              \
+  }
+
+SPFFT_STAGE_LAUNCHER(launch_z_backward, k_z_backward, nz, a.nst,
+                     dim3((a.nst + a.batch - 1) / a.batch))
+SPFFT_STAGE_LAUNCHER(launch_y_backward, k_y_backward, ny, a.ncol,
+                     dim3((a.ncol + a.batch - 1) / a.batch, a.nz))
+SPFFT_STAGE_LAUNCHER(launch_x_backward, k_x_backward, nx, a.ny,
+                     dim3((a.ny + a.batch - 1) / a.batch, a.nz))
+SPFFT_STAGE_LAUNCHER(launch_x_forward, k_x_forward, nx, a.ny,
+                     dim3((a.ny + a.batch - 1) / a.batch, a.nz))
+SPFFT_STAGE_LAUNCHER(launch_y_forward, k_y_forward, ny, a.ncol,
+                     dim3((a.ncol + a.batch - 1) / a.batch, a.nz))
+SPFFT_STAGE_LAUNCHER(launch_z_forward, k_z_forward, nz, a.nst,
+                     dim3((a.nst + a.batch - 1) / a.batch))
+
+#undef SPFFT_STAGE_LAUNCHER
+
+#define SPFFT_INSTANTIATE(T)                                                      \
+  template int stage_batch<T>(int, int, size_t*);                                \
+  template void generate_twiddles<T>(Cx<T>*, int, hipStream_t);                  \
+  template void launch_z_backward<T>(const StageArgs<T>&, hipStream_t);          \
+  template void launch_y_backward<T>(const StageArgs<T>&, hipStream_t);          \
+  template void launch_x_backward<T>(const StageArgs<T>&, hipStream_t);          \
+  template void launch_x_forward<T>(const StageArgs<T>&, hipStream_t);           \
+  template void launch_y_forward<T>(const StageArgs<T>&, hipStream_t);           \
+  template void launch_z_forward<T>(const StageArgs<T>&, hipStream_t);
+
+SPFFT_INSTANTIATE(double)
+SPFFT_INSTANTIATE(float)
+
+}  // namespace spfft

@@ -1,0 +1,94 @@
+#include "transform_internal.hpp"
+
+#include "hip_check.hpp"
+#include "spfft/exceptions.hpp"
+
+namespace spfft {
+
+template <typename T>
+TransformInternal<T>::TransformInternal(SpfftProcessingUnitType processingUnit,
+                                        std::shared_ptr<GridInternal<T>> grid,
+                                        std::shared_ptr<IndexMap> map)
+    : processingUnit_(processingUnit), map_(std::move(map)), grid_(std::move(grid)) {
+  // Checks in the order of the reference (transform_internal.cpp:52-77).
+  if (!grid_) throw InvalidParameterError();
+  if (map_->dimZ > grid_->max_local_z_length()) throw InvalidParameterError();
+  if (map_->num_sticks() > grid_->max_num_local_z_columns()) throw InvalidParameterError();
+  if (map_->dimX > grid_->max_dim_x() || map_->dimY > grid_->max_dim_y() ||
+      map_->dimZ > grid_->max_dim_z()) {
+    throw InvalidParameterError();
+  }
+  if (!(processingUnit & grid_->processing_unit())) throw InvalidParameterError();
+  if (processingUnit != SPFFT_PU_HOST && processingUnit != SPFFT_PU_GPU) {
+    throw InvalidParameterError();
+  }
+  if (map_->dimX <= 0 || map_->dimY <= 0 || map_->dimZ <= 0) throw InvalidParameterError();
+  if (processingUnit == SPFFT_PU_HOST) {
+    // This library executes on the GPU only; host execution is the
+    // reference's FFTW path and is not provided here.
+    throw HostExecutionError();
+  }
+  DeviceGuard guard(grid_->device_id());
+  exec_.reset(new ExecutionGPU<T>(map_, grid_));
+}
+
+template <typename T>
+TransformInternal<T> TransformInternal<T>::clone() const {
+  std::shared_ptr<GridInternal<T>> newGrid(new GridInternal<T>(*grid_));
+  TransformInternal<T> t(processingUnit_, std::move(newGrid), map_);
+  t.execMode_ = execMode_;
+  return t;
+}
+
+template <typename T>
+T* TransformInternal<T>::space_domain_data(SpfftProcessingUnitType location) {
+  return location == SPFFT_PU_GPU ? exec_->space_domain_gpu() : exec_->space_domain_host();
+}
+
+template <typename T>
+void TransformInternal<T>::forward(SpfftProcessingUnitType inputLocation, T* output,
+                                   SpfftScalingType scaling) {
+  forward(space_domain_data(inputLocation), output, scaling);
+}
+
+template <typename T>
+void TransformInternal<T>::forward(const T* input, T* output, SpfftScalingType scaling) {
+  DeviceGuard guard(grid_->device_id());
+  exec_->forward(input, output, scaling);
+  exec_->synchronize(execMode_);
+}
+
+template <typename T>
+void TransformInternal<T>::backward(const T* input, SpfftProcessingUnitType outputLocation) {
+  backward(input, space_domain_data(outputLocation));
+}
+
+template <typename T>
+void TransformInternal<T>::backward(const T* input, T* output) {
+  DeviceGuard guard(grid_->device_id());
+  exec_->backward(input, output);
+  exec_->synchronize(execMode_);
+}
+
+template <typename T>
+void TransformInternal<T>::forward_enqueue(const T* input, T* output, SpfftScalingType scaling) {
+  DeviceGuard guard(grid_->device_id());
+  exec_->forward(input, output, scaling);
+}
+
+template <typename T>
+void TransformInternal<T>::backward_enqueue(const T* input, T* output) {
+  DeviceGuard guard(grid_->device_id());
+  exec_->backward(input, output);
+}
+
+template <typename T>
+void TransformInternal<T>::synchronize() {
+  DeviceGuard guard(grid_->device_id());
+  exec_->synchronize(execMode_);
+}
+
+template class TransformInternal<double>;
+template class TransformInternal<float>;
+
+}  // namespace spfft

@@ -1,0 +1,266 @@
+"""GPU parity: the HIP path through the C ABI against the pinned CPU oracle.
+
+Bars (written per test):
+* FP64: relative L2 <= 1e-12 against the oracle (SPFFT_PU_HOST restatement);
+  data movement (backward twice, host vs device buffers) bit-exact.
+* FP32: relative L2 <= 1e-5 against the FP64 oracle.
+Matrix of the reference's local test (tests/local_tests/test_local_transform.cpp:95-111):
+sizes {1, 2, 11, 12, 13, 100} per dimension, centred and non-centred,
+C2C backward, C2C forward, R2C forward+backward.
+"""
+import numpy as np
+import pytest
+
+import spfft_amd as sp
+from oracle import oracle as orc
+from tests.workloads import complex_values, random_index_set, sphere_triplets, uniform
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-5
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def make(dims, t, r2c, precision="double"):
+    cls = sp.Transform if precision == "double" else sp.TransformFloat
+    return cls(-1, sp.SPFFT_PU_GPU, sp.SPFFT_TRANS_R2C if r2c else sp.SPFFT_TRANS_C2C, *dims,
+               len(t), sp.SPFFT_INDEX_TRIPLETS, t)
+
+
+def rdt(precision):
+    return np.float64 if precision == "double" else np.float32
+
+
+def gpu_backward(tr, values, dims, r2c, precision="double"):
+    nx, ny, nz = dims
+    v = dev(np.asarray(values, np.complex128).view(np.float64).astype(rdt(precision)))
+    n = nx * ny * nz * (1 if r2c else 2)
+    out = torch.empty(n, dtype=v.dtype, device="cuda")
+    tr.backward(v.data_ptr(), out.data_ptr())
+    o = out.cpu().numpy().astype(np.float64)
+    return o.reshape(nz, ny, nx) if r2c else o.view(np.complex128).reshape(nz, ny, nx)
+
+
+def gpu_forward(tr, space, r2c, scaling, precision="double"):
+    s = np.asarray(space)
+    flat = (s.astype(np.float64) if r2c else s.astype(np.complex128).view(np.float64)).ravel()
+    d = dev(flat.astype(rdt(precision)))
+    out = torch.empty(2 * tr_num_values(tr), dtype=d.dtype, device="cuda")
+    tr.forward(d.data_ptr(), out.data_ptr(), scaling)
+    return out.cpu().numpy().astype(np.float64).view(np.complex128)
+
+
+_NV = {}
+
+
+def tr_num_values(tr):
+    return _NV[id(tr)]
+
+
+def build(dims, t, r2c, precision="double"):
+    tr = make(dims, t, r2c, precision)
+    _NV[id(tr)] = len(t)
+    return tr
+
+
+SIZES = [1, 2, 11, 12, 13, 100]
+MATRIX = [(a, b, c) for a in SIZES for b in SIZES for c in SIZES]
+
+
+@pytest.mark.parametrize("centered", [False, True])
+@pytest.mark.parametrize("dims", MATRIX, ids=lambda d: "x".join(map(str, d)))
+def test_reference_matrix_c2c(dims, centered):
+    t = random_index_set(42, dims, False, centered=centered)
+    if len(t) == 0:
+        t = np.zeros((1, 3), np.int32)
+    v = complex_values(43, len(t))
+    o = orc.OraclePlan(False, dims, t)
+    tr = build(dims, t, False)
+    got = gpu_backward(tr, v, dims, False)
+    want = o.backward(v)
+    assert orc.rel_l2(got, want) <= FP64_TOL
+    again = gpu_backward(tr, v, dims, False)  # buffers re-zeroed on every call
+    np.testing.assert_array_equal(again, got)
+    nx, ny, nz = dims
+    s = complex_values(44, nx * ny * nz).reshape(nz, ny, nx)
+    for scaling in (sp.SPFFT_NO_SCALING, sp.SPFFT_FULL_SCALING):
+        f = gpu_forward(tr, s, False, scaling)
+        assert orc.rel_l2(f, o.forward(s, scaling == sp.SPFFT_FULL_SCALING)) <= FP64_TOL
+
+
+@pytest.mark.parametrize("centered", [False, True])
+@pytest.mark.parametrize("dims", MATRIX, ids=lambda d: "x".join(map(str, d)))
+def test_reference_matrix_r2c(dims, centered):
+    t = random_index_set(45, dims, True, centered=centered)
+    if len(t) == 0:
+        t = np.zeros((1, 3), np.int32)
+    nx, ny, nz = dims
+    o = orc.OraclePlan(True, dims, t)
+    tr = build(dims, t, True)
+    # the reference's R2C test: forward of a real field, then backward
+    s = uniform(46, nx * ny * nz).reshape(nz, ny, nx)
+    f = gpu_forward(tr, s, True, sp.SPFFT_NO_SCALING)
+    fo = o.forward(s, False)
+    assert orc.rel_l2(f, fo) <= FP64_TOL
+    b = gpu_backward(tr, fo, dims, True)
+    assert orc.rel_l2(b, o.backward(fo)) <= FP64_TOL
+    # arbitrary (non-hermitian) half-space input: the symmetry fill decides
+    v = complex_values(47, len(t))
+    assert orc.rel_l2(gpu_backward(tr, v, dims, True), o.backward(v)) <= FP64_TOL
+
+
+SPHERES = [
+    ("cfg1_32_r12_c2c", (32, 32, 32), 12, False),
+    ("cfg2a_128_r40_c2c", (128, 128, 128), 40, False),
+    ("cfg2b_128_r40_r2c", (128, 128, 128), 40, True),
+    ("96_r30_c2c", (96, 96, 96), 30, False),
+    ("192_r60_c2c", (192, 192, 192), 60, False),
+]
+
+
+@pytest.mark.parametrize("name,dims,radius,r2c", SPHERES, ids=[s[0] for s in SPHERES])
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_sphere_configs(name, dims, radius, r2c, precision):
+    tol = FP64_TOL if precision == "double" else FP32_TOL
+    t = sphere_triplets(radius, half=r2c)
+    v = complex_values(0x5EED0000 + len(t) % 97, len(t))
+    o = orc.OraclePlan(r2c, dims, t)
+    tr = build(dims, t, r2c, precision)
+    want = o.backward(v)
+    got = gpu_backward(tr, v, dims, r2c, precision)
+    err_b = orc.rel_l2(got, want)
+    f = gpu_forward(tr, want, r2c, sp.SPFFT_FULL_SCALING, precision)
+    err_f = orc.rel_l2(f, o.forward(want, True))
+    print(f"{name} {precision}: backward rel L2 {err_b:.3e}, forward rel L2 {err_f:.3e}")
+    assert err_b <= tol and err_f <= tol
+
+
+def test_host_pointers_match_device_pointers():
+    dims = (12, 11, 13)
+    t = random_index_set(61, dims, False, centered=True)
+    v = complex_values(62, len(t))
+    tr = build(dims, t, False)
+    d = gpu_backward(tr, v, dims, False)
+    host_out = np.zeros(2 * 12 * 11 * 13)
+    tr.backward(np.ascontiguousarray(v.view(np.float64)), host_out)
+    np.testing.assert_array_equal(host_out.view(np.complex128).reshape(13, 11, 12), d)
+    fwd_host = np.zeros(2 * len(t))
+    tr.forward(np.ascontiguousarray(d.view(np.float64).ravel()), fwd_host, sp.SPFFT_FULL_SCALING)
+    fwd_dev = gpu_forward(tr, d, False, sp.SPFFT_FULL_SCALING)
+    np.testing.assert_array_equal(fwd_host.view(np.complex128), fwd_dev)
+
+
+def test_space_domain_buffers():
+    dims = (16, 12, 10)
+    t = random_index_set(63, dims, False)
+    v = complex_values(64, len(t))
+    tr = build(dims, t, False)
+    d_in = dev(v.view(np.float64))
+    tr.backward(d_in.data_ptr(), sp.SPFFT_PU_GPU)
+    out = torch.empty(2 * len(t), dtype=torch.float64, device="cuda")
+    tr.forward(sp.SPFFT_PU_GPU, out.data_ptr(), sp.SPFFT_FULL_SCALING)
+    assert orc.rel_l2(out.cpu().numpy().view(np.complex128), v) <= FP64_TOL
+    # host mirror
+    host = np.zeros(2 * len(t))
+    tr.backward(np.ascontiguousarray(v.view(np.float64)), sp.SPFFT_PU_HOST)
+    tr.forward(sp.SPFFT_PU_HOST, host, sp.SPFFT_FULL_SCALING)
+    assert orc.rel_l2(host.view(np.complex128), v) <= FP64_TOL
+
+
+def test_async_mode_orders_default_stream():
+    dims = (32, 32, 32)
+    t = sphere_triplets(12)
+    v = complex_values(65, len(t))
+    tr = build(dims, t, False)
+    tr.set_execution_mode(sp.SPFFT_EXEC_ASYNCHRONOUS)
+    assert tr.execution_mode() == sp.SPFFT_EXEC_ASYNCHRONOUS
+    d_in = dev(v.view(np.float64))
+    out = torch.empty(2 * 32 ** 3, dtype=torch.float64, device="cuda")
+    tr.backward(d_in.data_ptr(), out.data_ptr())
+    res = out.cpu().numpy().view(np.complex128).reshape(32, 32, 32)  # default-stream copy
+    assert orc.rel_l2(res, orc.OraclePlan(False, dims, t).backward(v)) <= FP64_TOL
+
+
+def test_multi_transform_known_answer():
+    # reference tests/mpi_tests/test_multi_transform.cpp:52-89: values (i, i),
+    # backward then unscaled forward gives i * N.
+    dims = (12, 13, 11)
+    t = random_index_set(66, dims, False, centered=True)
+    base = build(dims, t, False)
+    trs = [base, base.clone(), base.clone()]
+    n = len(t)
+    N = 12 * 13 * 11
+    ins, outs, spaces = [], [], []
+    for k in range(3):
+        vals = (np.arange(n) + k) * (1 + 1j)
+        ins.append(dev(vals.view(np.float64)))
+        spaces.append(torch.empty(2 * N, dtype=torch.float64, device="cuda"))
+        outs.append(torch.empty(2 * n, dtype=torch.float64, device="cuda"))
+    sp.multi_transform_backward(trs, ins, spaces)
+    sp.multi_transform_forward(trs, spaces, outs, [sp.SPFFT_NO_SCALING] * 3)
+    for k in range(3):
+        got = outs[k].cpu().numpy().view(np.complex128)
+        want = (np.arange(n) + k) * (1 + 1j) * N
+        assert np.max(np.abs(got - want)) <= 1e-8 * max(1.0, np.max(np.abs(want)))
+    # locations form
+    sp.multi_transform_backward(trs, ins, [sp.SPFFT_PU_GPU] * 3)
+    sp.multi_transform_forward(trs, [sp.SPFFT_PU_GPU] * 3, outs, [sp.SPFFT_FULL_SCALING] * 3)
+    for k in range(3):
+        got = outs[k].cpu().numpy().view(np.complex128)
+        assert orc.rel_l2(got, (np.arange(n) + k) * (1 + 1j)) <= FP64_TOL
+
+
+def test_multi_transform_rejects_shared_grid():
+    dims = (8, 8, 8)
+    t = random_index_set(67, dims, False)
+    g = sp.Grid(8, 8, 8, 64, sp.SPFFT_PU_GPU, -1)
+    a = g.create_transform(sp.SPFFT_PU_GPU, sp.SPFFT_TRANS_C2C, 8, 8, 8, 8, len(t),
+                           sp.SPFFT_INDEX_TRIPLETS, t)
+    b = g.create_transform(sp.SPFFT_PU_GPU, sp.SPFFT_TRANS_C2C, 8, 8, 8, 8, len(t),
+                           sp.SPFFT_INDEX_TRIPLETS, t)
+    x = torch.zeros(2 * len(t), dtype=torch.float64, device="cuda")
+    with pytest.raises(sp.SpfftError) as e:
+        sp.multi_transform_backward([a, b], [x, x], [sp.SPFFT_PU_GPU] * 2)
+    assert e.value.code == sp.SPFFT_INVALID_PARAMETER_ERROR
+
+
+def test_grid_limits_and_accessors():
+    g = sp.Grid(16, 12, 10, 50, sp.SPFFT_PU_GPU, 3)
+    assert (g.max_dim_x(), g.max_dim_y(), g.max_dim_z()) == (16, 12, 10)
+    assert g.max_num_local_z_columns() == 50 and g.num_threads() == 3
+    assert g.processing_unit() == sp.SPFFT_PU_GPU
+    t = random_index_set(68, (16, 12, 10), False, stick_fraction=0.2)
+    tr = g.create_transform(sp.SPFFT_PU_GPU, sp.SPFFT_TRANS_C2C, 16, 12, 10, 10, len(t),
+                            sp.SPFFT_INDEX_TRIPLETS, t)
+    assert (tr.dim_x(), tr.dim_y(), tr.dim_z(), tr.local_z_length()) == (16, 12, 10, 10)
+    assert tr.global_size() == 1920 and tr.num_global_elements() == len(t)
+    assert tr.local_slice_size() == 1920 and tr.local_z_offset() == 0
+    assert tr.num_local_elements() == 10  # reports the z length, as the reference does
+    with pytest.raises(sp.SpfftError) as e:  # larger than the grid
+        g.create_transform(sp.SPFFT_PU_GPU, sp.SPFFT_TRANS_C2C, 17, 12, 10, 10, 1,
+                           sp.SPFFT_INDEX_TRIPLETS, [[0, 0, 0]])
+    assert e.value.code == sp.SPFFT_INVALID_PARAMETER_ERROR
+    many = random_index_set(69, (16, 12, 10), False, stick_fraction=1.0)
+    with pytest.raises(sp.SpfftError) as e:  # more columns than the grid allows
+        g.create_transform(sp.SPFFT_PU_GPU, sp.SPFFT_TRANS_C2C, 16, 12, 10, 10, len(many),
+                           sp.SPFFT_INDEX_TRIPLETS, many)
+    assert e.value.code == sp.SPFFT_INVALID_PARAMETER_ERROR
+
+
+def test_examples_run():
+    import os
+    import subprocess
+    from tests.test_abi import test_examples_compile
+    test_examples_compile()
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, LD_LIBRARY_PATH=os.path.dirname(sp.LIB_PATH))
+    for exe in ("example_c", "example_cpp"):
+        r = subprocess.run([os.path.join(root, "tests", "examples", "bin", exe)], env=env, capture_output=True,
+                           text=True, timeout=60)
+        print(r.stdout, r.stderr)
+        assert r.returncode == 0, exe
