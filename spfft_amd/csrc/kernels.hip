@@ -13,6 +13,7 @@
 
 #include "hip_check.hpp"
 #include "kernels.hpp"
+#include "profiling.hpp"
 
 namespace spfft {
 
@@ -504,6 +505,7 @@ void generate_twiddles(Cx<T>* tw, int n, hipStream_t stream) {
     (void)stage_batch<T>(a.LEN, COUNT, &lds);                                    \
     lds = 2 * static_cast<size_t>(a.LEN) * sizeof(Cx<T>) * a.batch;              \
     launch_stage(KERNEL<T>, GRID, lds, stream, &a);                              \
+    prof::KernelSpan span(stream, #KERNEL);                                      \
     hipLaunchKernelGGL(KERNEL<T>, GRID, dim3(kBlock), lds, stream, a);           \
     check_launch();                                                              \
   }

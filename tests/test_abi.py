@@ -32,7 +32,7 @@ def declared_functions():
 
 def test_every_declared_symbol_is_exported():
     names = declared_functions()
-    assert len(names) == 73  # 36 double + 36 float + index-map hook
+    assert len(names) == 75  # 36 double + 36 float + index-map and profiling hooks
     out = subprocess.run(["nm", "-D", "--defined-only", sp.LIB_PATH], check=True,
                          capture_output=True, text=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
