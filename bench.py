@@ -41,13 +41,13 @@ def kernel_bytes(name, ns, ng, r2c, real_bytes):
 
     Per direction the compulsory traffic is values (2 reals per value), the
     int32 value->slot map and the dense space grid. The z kernels own the
-    value and map terms, the x kernels own the dense grid; the y kernels only
-    move intermediates and own none."""
+    value and map terms, the x / xy kernels own the dense grid; the generic
+    path's y kernels only move intermediates and own none."""
     cplx = 2 * real_bytes
     grid = ng * (real_bytes if r2c else cplx)
     if name.startswith("k_z_"):
         return ns * (cplx + 4)
-    if name.startswith("k_x_"):
+    if name.startswith("k_x_") or name.startswith("k_xy_"):
         return grid
     return 0
 
@@ -136,17 +136,25 @@ def main():
     torch.cuda.synchronize()
     log(f"warmup done ({args.warmup} pairs)")
 
-    sp.lib.spfft_amd_profiling_enable(1)
+    # Timed region: exactly --steps pairs, synchronised on both sides.
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         pair(i)
+    t_enq = time.perf_counter() - t0
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    value = args.steps / elapsed
+    log(f"timed {args.steps} pairs in {elapsed * 1e3:.1f} ms -> {value:.1f} pairs/s "
+        f"(host enqueue {t_enq / args.steps * 1e6:.1f} us/pair)")
+
+    # Same steps again with per-kernel HIP events on the library's stream.
+    sp.lib.spfft_amd_profiling_enable(1)
+    for i in range(args.steps):
+        pair(i)
+    torch.cuda.synchronize()
     sp.lib.spfft_amd_profiling_enable(0)
     prof = read_profile(sp.lib)
-    value = args.steps / elapsed
-    log(f"timed {args.steps} pairs in {elapsed * 1e3:.1f} ms -> {value:.1f} pairs/s")
 
     kernels = {}
     for nm, (cnt, tot) in prof.items():

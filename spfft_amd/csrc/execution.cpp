@@ -1,6 +1,8 @@
 #include "execution.hpp"
 
+#include <cstdlib>
 #include <utility>
+#include <vector>
 
 #include "hip_check.hpp"
 #include "spfft/exceptions.hpp"
@@ -25,6 +27,13 @@ std::pair<P*, P*> translate_pointer(P* ptr) {
     return {nullptr, static_cast<P*>(attr.devicePointer)};
   }
   return {attr.hostPointer ? static_cast<P*>(attr.hostPointer) : ptr, nullptr};
+}
+
+// SPFFT_AMD_GENERIC=1 forces the generic stage kernels (used to compare the
+// two paths).
+bool disable_fast_path() {
+  const char* v = std::getenv("SPFFT_AMD_GENERIC");
+  return v && v[0] == '1';
 }
 
 }  // namespace
@@ -59,6 +68,34 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
   yBatch_ = stage_batch<T>(map_->dimY, std::max(1, numCols_), &lds);
   xBatch_ = stage_batch<T>(map_->dimX, map_->dimY, &lds);
 
+  // Fixed-length C2C kernels when the lengths have compile-time schedules.
+  fast_ = map_->type == SPFFT_TRANS_C2C && !disable_fast_path() &&
+          fast_path_supported<T>(map_->dimX, map_->dimY, map_->dimZ, numCols_);
+  if (fast_) {
+    const int ny = map_->dimY;
+    std::vector<int16_t> rel(static_cast<size_t>(numCols_) * ny, int16_t(-1));
+    for (int s = 0; s < map_->num_sticks(); ++s) {
+      const int u = t.stickCol[s];
+      rel[static_cast<size_t>(u) * ny + t.stickY[s]] = static_cast<int16_t>(s - t.colBegin[u]);
+    }
+    stickRel_ = DeviceBuffer<int16_t>::from_host(rel);
+    const int classes = fast_classes(map_->dimX);
+    std::vector<int> resBegin(classes + 1, 0), resSticks;
+    resSticks.reserve(map_->num_sticks());
+    for (int c = 0; c < classes; ++c) {
+      resBegin[c] = static_cast<int>(resSticks.size());
+      for (int s = 0; s < map_->num_sticks(); ++s) {
+        if (t.colX[t.stickCol[s]] % classes == c) resSticks.push_back(s);
+      }
+    }
+    resBegin[classes] = static_cast<int>(resSticks.size());
+    std::vector<int> xCol(map_->dimX, -1);
+    for (int u = 0; u < numCols_; ++u) xCol[t.colX[u]] = u;
+    xCol_ = DeviceBuffer<int>::from_host(xCol);
+    resSticks_ = DeviceBuffer<int>::from_host(resSticks);
+    resBegin_ = DeviceBuffer<int>::from_host(resBegin);
+  }
+
   twX_ = DeviceBuffer<Cx<T>>(map_->dimX);
   twY_ = DeviceBuffer<Cx<T>>(map_->dimY);
   twZ_ = DeviceBuffer<Cx<T>>(map_->dimZ);
@@ -83,12 +120,14 @@ size_t ExecutionGPU<T>::space_domain_size() const {
 }
 
 template <typename T>
-void ExecutionGPU<T>::begin() {
+hipStream_t ExecutionGPU<T>::begin(bool concurrent) {
   // An error left by earlier GPU work is reported, not absorbed
   // (reference execution_gpu.cpp:256-258).
   if (hipGetLastError() != hipSuccess) throw GPUPrecedingError();
+  if (!concurrent) return nullptr;
   check_hip(hipEventRecord(startEvent_, nullptr));
   check_hip(hipStreamWaitEvent(stream_, startEvent_, 0));
+  return stream_;
 }
 
 template <typename T>
@@ -116,8 +155,33 @@ StageArgs<T> ExecutionGPU<T>::base_args() const {
 }
 
 template <typename T>
-void ExecutionGPU<T>::backward(const T* input, T* output) {
-  begin();
+FastArgs<T> ExecutionGPU<T>::fast_args() const {
+  FastArgs<T> a;
+  a.nz = map_->dimZ;
+  a.nst = map_->num_sticks();
+  a.ncol = numCols_;
+  a.zbuf = reinterpret_cast<Cx<T>*>(grid_->sticks_gpu());
+  a.entryValue = entryValue_.data();
+  a.entrySlot = entrySlot_.data();
+  a.stickBegin = stickBegin_.data();
+  a.stickY = stickY_.data();
+  a.stickCol = stickCol_.data();
+  a.colX = colX_.data();
+  a.colBegin = colBegin_.data();
+  a.stickRel = stickRel_.data();
+  a.resSticks = resSticks_.data();
+  a.resBegin = resBegin_.data();
+  a.xCol = xCol_.data();
+  a.twX = twX_.data();
+  a.twY = twY_.data();
+  a.twZ = twZ_.data();
+  a.scale = scale_;
+  return a;
+}
+
+template <typename T>
+void ExecutionGPU<T>::backward(const T* input, T* output, bool concurrent) {
+  hipStream_t stream = begin(concurrent);
   const size_t numValueReals = 2 * static_cast<size_t>(map_->num_values());
   auto in = translate_pointer(input);
   auto out = translate_pointer(output);
@@ -125,10 +189,23 @@ void ExecutionGPU<T>::backward(const T* input, T* output) {
   if (numValueReals > 0 && !inGPU) {
     // Host input: stage in the work buffer, which is free until the y stage.
     check_hip(hipMemcpyAsync(grid_->work_gpu(), in.first, numValueReals * sizeof(T),
-                             hipMemcpyHostToDevice, stream_));
+                             hipMemcpyHostToDevice, stream));
     inGPU = grid_->work_gpu();
   }
   T* outGPU = out.second ? out.second : grid_->space_gpu();
+
+  if (fast_) {
+    FastArgs<T> f = fast_args();
+    f.in = inGPU;
+    f.out = outGPU;
+    launch_fast_z_backward<T>(map_->dimZ, f, stream);
+    launch_fast_xy_backward<T>(map_->dimX, map_->dimY, f, stream);
+    if (out.first) {
+      check_hip(hipMemcpyAsync(out.first, outGPU, space_domain_size() * sizeof(T),
+                               hipMemcpyDeviceToHost, stream));
+    }
+    return;
+  }
 
   StageArgs<T> a = base_args();
   a.in = inGPU;
@@ -137,39 +214,54 @@ void ExecutionGPU<T>::backward(const T* input, T* output) {
     a.fft = fftZ_;
     a.tw = twZ_.data();
     a.batch = zBatch_;
-    launch_z_backward<T>(a, stream_);
+    launch_z_backward<T>(a, stream);
     a.fft = fftY_;
     a.tw = twY_.data();
     a.batch = yBatch_;
-    launch_y_backward<T>(a, stream_);
+    launch_y_backward<T>(a, stream);
   }
   a.fft = fftX_;
   a.tw = twX_.data();
   a.batch = xBatch_;
-  launch_x_backward<T>(a, stream_);
+  launch_x_backward<T>(a, stream);
 
   if (out.first) {
     check_hip(hipMemcpyAsync(out.first, outGPU, space_domain_size() * sizeof(T),
-                             hipMemcpyDeviceToHost, stream_));
+                             hipMemcpyDeviceToHost, stream));
   }
 }
 
 template <typename T>
-void ExecutionGPU<T>::forward(const T* input, T* output, SpfftScalingType scaling) {
-  begin();
+void ExecutionGPU<T>::forward(const T* input, T* output, SpfftScalingType scaling,
+                              bool concurrent) {
+  hipStream_t stream = begin(concurrent);
   const size_t numValueReals = 2 * static_cast<size_t>(map_->num_values());
   auto in = translate_pointer(input);
   auto out = translate_pointer(output);
   const T* inGPU = in.second;
   if (!inGPU) {
     check_hip(hipMemcpyAsync(grid_->space_gpu(), in.first, space_domain_size() * sizeof(T),
-                             hipMemcpyHostToDevice, stream_));
+                             hipMemcpyHostToDevice, stream));
     inGPU = grid_->space_gpu();
   }
   if (numValueReals == 0) return;
   // Host output: the z stage writes into the work buffer, which is free again
   // once the y stage has consumed it.
   T* outGPU = out.second ? out.second : grid_->work_gpu();
+
+  if (fast_) {
+    FastArgs<T> f = fast_args();
+    f.in = inGPU;
+    f.out = outGPU;
+    f.useScale = scaling == SPFFT_FULL_SCALING ? 1 : 0;
+    launch_fast_xy_forward<T>(map_->dimX, map_->dimY, f, stream);
+    launch_fast_z_forward<T>(map_->dimZ, f, stream);
+    if (out.first) {
+      check_hip(hipMemcpyAsync(out.first, outGPU, numValueReals * sizeof(T),
+                               hipMemcpyDeviceToHost, stream));
+    }
+    return;
+  }
 
   StageArgs<T> a = base_args();
   a.in = inGPU;
@@ -178,24 +270,28 @@ void ExecutionGPU<T>::forward(const T* input, T* output, SpfftScalingType scalin
   a.fft = fftX_;
   a.tw = twX_.data();
   a.batch = xBatch_;
-  launch_x_forward<T>(a, stream_);
+  launch_x_forward<T>(a, stream);
   a.fft = fftY_;
   a.tw = twY_.data();
   a.batch = yBatch_;
-  launch_y_forward<T>(a, stream_);
+  launch_y_forward<T>(a, stream);
   a.fft = fftZ_;
   a.tw = twZ_.data();
   a.batch = zBatch_;
-  launch_z_forward<T>(a, stream_);
+  launch_z_forward<T>(a, stream);
 
   if (out.first) {
     check_hip(hipMemcpyAsync(out.first, outGPU, numValueReals * sizeof(T), hipMemcpyDeviceToHost,
-                             stream_));
+                             stream));
   }
 }
 
 template <typename T>
-void ExecutionGPU<T>::synchronize(SpfftExecType mode) {
+void ExecutionGPU<T>::synchronize(SpfftExecType mode, bool concurrent) {
+  if (!concurrent) {
+    if (mode != SPFFT_EXEC_ASYNCHRONOUS) check_hip(hipStreamSynchronize(nullptr));
+    return;
+  }
   if (mode == SPFFT_EXEC_ASYNCHRONOUS) {
     check_hip(hipEventRecord(endEvent_, stream_));
     check_hip(hipStreamWaitEvent(nullptr, endEvent_, 0));

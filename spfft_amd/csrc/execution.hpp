@@ -9,6 +9,7 @@
 
 #include "grid_internal.hpp"
 #include "index_map.hpp"
+#include "fast_kernels.hpp"
 #include "kernels.hpp"
 #include "memory.hpp"
 #include "spfft/types.h"
@@ -23,13 +24,18 @@ public:
   ExecutionGPU(const ExecutionGPU&) = delete;
   ExecutionGPU& operator=(const ExecutionGPU&) = delete;
 
-  // Enqueue on the transform's stream, ordered after the default stream.
-  // Pointers may be device or host memory.
-  void backward(const T* input, T* output);
-  void forward(const T* input, T* output, SpfftScalingType scaling);
-  // SYNCHRONOUS: wait for the stream. ASYNCHRONOUS: make the default stream
-  // wait for it (reference execution_gpu.cpp:403-410).
-  void synchronize(SpfftExecType mode);
+  // Enqueue a transform. Pointers may be device or host memory.
+  // Single calls run on the default (null) stream itself, which orders them
+  // after earlier default-stream work and makes later default-stream work
+  // wait for them -- the reference's contract (execution_gpu.cpp:333-344,
+  // 403-410) without cross-stream hops. Batched calls (concurrent = true) run
+  // on the transform's own stream, fenced against the default stream with
+  // events, so independent transforms overlap.
+  void backward(const T* input, T* output, bool concurrent = false);
+  void forward(const T* input, T* output, SpfftScalingType scaling, bool concurrent = false);
+  // SYNCHRONOUS: wait for the work. ASYNCHRONOUS: return; a concurrent call
+  // additionally makes the default stream wait for its stream.
+  void synchronize(SpfftExecType mode, bool concurrent = false);
 
   T* space_domain_gpu() const { return grid_->space_gpu(); }
   T* space_domain_host() const { return grid_->space_host(); }
@@ -38,8 +44,10 @@ public:
   size_t space_domain_size() const;
 
 private:
-  void begin();
+  // Checks for preceding errors and returns the stream to enqueue on.
+  hipStream_t begin(bool concurrent);
   StageArgs<T> base_args() const;
+  FastArgs<T> fast_args() const;
 
   std::shared_ptr<IndexMap> map_;
   std::shared_ptr<GridInternal<T>> grid_;
@@ -51,6 +59,10 @@ private:
   FftDesc fftX_, fftY_, fftZ_;
   DeviceBuffer<Cx<T>> twX_, twY_, twZ_;
   DeviceBuffer<int> entryValue_, entrySlot_, stickBegin_, stickY_, stickCol_, colX_, colBegin_;
+  // fixed-length C2C path (two kernels per direction)
+  bool fast_ = false;
+  DeviceBuffer<int16_t> stickRel_;
+  DeviceBuffer<int> resSticks_, resBegin_, xCol_;
 };
 
 }  // namespace spfft

@@ -1,0 +1,67 @@
+// Fixed-length fast path of the C2C sparse 3D FFT (lengths with compile-time
+// radix schedules). Two kernels per direction:
+//
+//   z kernel  : sparse values -> LDS sticks -> z FFT -> zbuf[z][stick]
+//               (forward: zbuf -> z FFT -> gather/scale into user order)
+//   xy kernel : one workgroup per (plane z, residue class), no intermediate
+//               between the y and x transforms.
+//     backward, class r of R: rows y = r + R m. Decimation in frequency over
+//       y: t_u[j] = W_NY^(r j) sum_l X_u[j + M l] W_R^(r l), then an M-point
+//       FFT per column gives the rows' y-transformed values; the rows are
+//       x-transformed in LDS and streamed to the dense plane.
+//     forward, class c of R: columns x = c + R m. Decimation in frequency
+//       over x on every row, M-point x FFTs, then full-length y FFTs on the
+//       class's columns; results are gathered at the stick positions.
+//   Each class reads the whole plane input (from L2 after the first class);
+//   the classes of a plane are mapped to the same XCD.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace spfft {
+
+template <typename T>
+struct FastArgs {
+  int nz = 0, nst = 0, ncol = 0;
+  Cx<T>* zbuf = nullptr;
+  const T* in = nullptr;
+  T* out = nullptr;
+  const int* entryValue = nullptr;
+  const int* entrySlot = nullptr;
+  const int* stickBegin = nullptr;
+  const int* stickY = nullptr;
+  const int* stickCol = nullptr;
+  const int* colX = nullptr;
+  const int* colBegin = nullptr;
+  const int16_t* stickRel = nullptr;  // [ncol][NY]: stick index within its column, or -1
+  const int* xCol = nullptr;          // [NX]: column id of x, or -1
+  const int* resSticks = nullptr;     // sticks grouped by colX mod R
+  const int* resBegin = nullptr;      // R + 1 offsets into resSticks
+  const Cx<T>* twX = nullptr;
+  const Cx<T>* twY = nullptr;
+  const Cx<T>* twZ = nullptr;
+  T scale = T(1);
+  int useScale = 0;
+};
+
+// Residue classes per plane used by the xy kernels of length n.
+constexpr int fast_classes(int n) { return n >= 192 ? 8 : 4; }
+
+// True when (nx, ny, nz) have fast kernels and their LDS fits for ncol columns.
+template <typename T>
+bool fast_path_supported(int nx, int ny, int nz, int ncol);
+
+template <typename T>
+void launch_fast_z_backward(int nz, const FastArgs<T>& a, hipStream_t stream);
+template <typename T>
+void launch_fast_z_forward(int nz, const FastArgs<T>& a, hipStream_t stream);
+template <typename T>
+void launch_fast_xy_backward(int nx, int ny, const FastArgs<T>& a, hipStream_t stream);
+template <typename T>
+void launch_fast_xy_forward(int nx, int ny, const FastArgs<T>& a, hipStream_t stream);
+
+}  // namespace spfft

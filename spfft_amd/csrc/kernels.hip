@@ -10,10 +10,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <unordered_map>
 
 #include "hip_check.hpp"
 #include "kernels.hpp"
 #include "profiling.hpp"
+#include "fft_common.hpp"
 
 namespace spfft {
 
@@ -21,145 +24,11 @@ extern __shared__ __attribute__((aligned(16))) unsigned char spfft_lds[];
 
 namespace {
 
+using namespace dev;
+
 constexpr int kBlock = 256;
 constexpr size_t kLdsDefault = 64 * 1024;
 constexpr size_t kLdsMax = 160 * 1024;
-
-template <typename T>
-__device__ __forceinline__ Cx<T> cadd(Cx<T> a, Cx<T> b) {
-  return {a.re + b.re, a.im + b.im};
-}
-template <typename T>
-__device__ __forceinline__ Cx<T> csub(Cx<T> a, Cx<T> b) {
-  return {a.re - b.re, a.im - b.im};
-}
-template <typename T>
-__device__ __forceinline__ Cx<T> cmul(Cx<T> a, Cx<T> b) {
-  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-template <typename T>
-__device__ __forceinline__ Cx<T> cscale(Cx<T> a, T s) {
-  return {a.re * s, a.im * s};
-}
-template <typename T>
-__device__ __forceinline__ Cx<T> cconj(Cx<T> a) {
-  return {a.re, -a.im};
-}
-template <typename T>
-__device__ __forceinline__ bool cnonzero(Cx<T> a) {
-  return a.re != T(0) || a.im != T(0);
-}
-// Multiply by -i (forward) or +i (backward).
-template <bool BWD, typename T>
-__device__ __forceinline__ Cx<T> rot(Cx<T> a) {
-  return BWD ? Cx<T>{-a.im, a.re} : Cx<T>{a.im, -a.re};
-}
-template <bool BWD, typename T>
-__device__ __forceinline__ Cx<T> twiddle(const Cx<T>* __restrict__ w, int t) {
-  const Cx<T> v = w[t];
-  return BWD ? cconj(v) : v;
-}
-
-// ---- butterflies: in-register DFT of length R with sign -1 (fwd) / +1 (bwd)
-
-template <typename T, int R, bool BWD>
-struct Butterfly {
-  // Direct DFT from the length-n table: W_R^m = W_n^(m n / R).
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>* __restrict__ w, int n) {
-    Cx<T> out[R];
-    const int step = n / R;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      Cx<T> acc = v[0];
-#pragma unroll
-      for (int r = 1; r < R; ++r) acc = cadd(acc, cmul(v[r], twiddle<BWD>(w, ((r * q) % R) * step)));
-      out[q] = acc;
-    }
-#pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = out[q];
-  }
-};
-
-template <typename T, bool BWD>
-struct Butterfly<T, 2, BWD> {
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
-    const Cx<T> a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-  }
-};
-
-template <typename T, bool BWD>
-struct Butterfly<T, 4, BWD> {
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
-    const Cx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-    const Cx<T> t2 = cadd(v[1], v[3]), t3 = rot<BWD>(csub(v[1], v[3]));
-    v[0] = cadd(t0, t2);
-    v[2] = csub(t0, t2);
-    v[1] = cadd(t1, t3);
-    v[3] = csub(t1, t3);
-  }
-};
-
-template <typename T, bool BWD>
-struct Butterfly<T, 8, BWD> {
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>* w, int n) {
-    Cx<T> e[4] = {v[0], v[2], v[4], v[6]};
-    Cx<T> o[4] = {v[1], v[3], v[5], v[7]};
-    Butterfly<T, 4, BWD>::run(e, w, n);
-    Butterfly<T, 4, BWD>::run(o, w, n);
-    const T h = T(0.70710678118654752440);
-    // o[k] *= exp(-/+ 2 pi i k / 8)
-    const Cx<T> o1 = BWD ? Cx<T>{h * (o[1].re - o[1].im), h * (o[1].re + o[1].im)}
-                         : Cx<T>{h * (o[1].re + o[1].im), h * (o[1].im - o[1].re)};
-    const Cx<T> o2 = rot<BWD>(o[2]);
-    const Cx<T> o3 = BWD ? Cx<T>{-h * (o[3].re + o[3].im), h * (o[3].re - o[3].im)}
-                         : Cx<T>{h * (o[3].im - o[3].re), -h * (o[3].re + o[3].im)};
-    v[0] = cadd(e[0], o[0]);
-    v[4] = csub(e[0], o[0]);
-    v[1] = cadd(e[1], o1);
-    v[5] = csub(e[1], o1);
-    v[2] = cadd(e[2], o2);
-    v[6] = csub(e[2], o2);
-    v[3] = cadd(e[3], o3);
-    v[7] = csub(e[3], o3);
-  }
-};
-
-template <typename T, bool BWD>
-struct Butterfly<T, 3, BWD> {
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
-    const T s3 = T(0.86602540378443864676);  // sin(2 pi / 3)
-    const Cx<T> t1 = cadd(v[1], v[2]);
-    const Cx<T> t2 = csub(v[1], v[2]);
-    const Cx<T> m = csub(v[0], cscale(t1, T(0.5)));
-    const Cx<T> r = rot<BWD>(cscale(t2, s3));
-    v[0] = cadd(v[0], t1);
-    v[1] = cadd(m, r);
-    v[2] = csub(m, r);
-  }
-};
-
-template <typename T, bool BWD>
-struct Butterfly<T, 5, BWD> {
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
-    const T c1 = T(0.30901699437494742410);   // cos(2 pi / 5)
-    const T c2 = T(-0.80901699437494742410);  // cos(4 pi / 5)
-    const T s1 = T(0.95105651629515357212);   // sin(2 pi / 5)
-    const T s2 = T(0.58778525229247312917);   // sin(4 pi / 5)
-    const Cx<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-    const Cx<T> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-    const Cx<T> b1 = cadd(v[0], cadd(cscale(t1, c1), cscale(t2, c2)));
-    const Cx<T> b2 = cadd(v[0], cadd(cscale(t1, c2), cscale(t2, c1)));
-    const Cx<T> d1 = rot<BWD>(cadd(cscale(t3, s1), cscale(t4, s2)));
-    const Cx<T> d2 = rot<BWD>(csub(cscale(t3, s2), cscale(t4, s1)));
-    v[0] = cadd(v[0], cadd(t1, t2));
-    v[1] = cadd(b1, d1);
-    v[4] = csub(b1, d1);
-    v[2] = cadd(b2, d2);
-    v[3] = csub(b2, d2);
-  }
-};
 
 // ---- one Stockham pass over `batch` contiguous lines of length n in LDS.
 // Ns = product of the radices already applied. Butterfly j reads
@@ -445,15 +314,22 @@ __global__ void k_twiddles(Cx<T>* tw, int n) {
   tw[t] = Cx<T>{static_cast<T>(c), static_cast<T>(s)};
 }
 
+// Raise the dynamic LDS limit of a kernel once per process.
 template <typename Kernel>
 void launch_stage(Kernel kernel, dim3 grid, size_t lds, hipStream_t stream,
                   const void* argsPtr) {
-  if (lds > kLdsDefault) {
-    check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  static_cast<int>(lds)));
-  }
+  (void)grid;
+  (void)stream;
   (void)argsPtr;
+  if (lds <= kLdsDefault) return;
+  static std::mutex mutex;
+  static std::unordered_map<const void*, bool> raised;
+  const void* key = reinterpret_cast<const void*>(kernel);
+  std::lock_guard<std::mutex> lock(mutex);
+  if (raised.count(key)) return;
+  check_hip(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(kLdsMax)));
+  raised[key] = true;
 }
 
 }  // namespace

@@ -73,19 +73,19 @@ void TransformInternal<T>::backward(const T* input, T* output) {
 template <typename T>
 void TransformInternal<T>::forward_enqueue(const T* input, T* output, SpfftScalingType scaling) {
   DeviceGuard guard(grid_->device_id());
-  exec_->forward(input, output, scaling);
+  exec_->forward(input, output, scaling, true);
 }
 
 template <typename T>
 void TransformInternal<T>::backward_enqueue(const T* input, T* output) {
   DeviceGuard guard(grid_->device_id());
-  exec_->backward(input, output);
+  exec_->backward(input, output, true);
 }
 
 template <typename T>
 void TransformInternal<T>::synchronize() {
   DeviceGuard guard(grid_->device_id());
-  exec_->synchronize(execMode_);
+  exec_->synchronize(execMode_, true);
 }
 
 template class TransformInternal<double>;

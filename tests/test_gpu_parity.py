@@ -264,3 +264,30 @@ def test_examples_run():
                            text=True, timeout=60)
         print(r.stdout, r.stderr)
         assert r.returncode == 0, exe
+
+
+FAST_LENGTHS = [32, 64, 96, 128, 192]
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("n", FAST_LENGTHS)
+@pytest.mark.parametrize("fill", [(0.7, 0.7, False), (0.15, 0.5, True), (1.0, 1.0, True)],
+                         ids=["dense70", "sparse_centred", "full"])
+def test_fixed_length_path_random_sets(n, fill, precision):
+    """Random index sets on the fixed-length C2C kernels (every column pattern,
+    including fully dense planes)."""
+    sf, ff, centered = fill
+    if n == 192 and sf == 1.0:
+        pytest.skip("dense 192^3 column set exceeds the fixed-length LDS budget (generic path)")
+    dims = (n, n, n)
+    t = random_index_set(70 + n, dims, False, stick_fraction=sf, fill_fraction=ff,
+                         centered=centered)
+    t = t[np.argsort(uniform(71, len(t)))]  # arbitrary value order
+    v = complex_values(72, len(t))
+    o = orc.OraclePlan(False, dims, t)
+    tr = build(dims, t, False, precision)
+    tol = FP64_TOL if precision == "double" else FP32_TOL
+    want = o.backward(v)
+    assert orc.rel_l2(gpu_backward(tr, v, dims, False, precision), want) <= tol
+    f = gpu_forward(tr, want, False, sp.SPFFT_FULL_SCALING, precision)
+    assert orc.rel_l2(f, o.forward(want, True)) <= tol
