@@ -86,6 +86,52 @@ def cpu_baseline(t, dims, values, seconds):
                       f"({el:.1f} s), C/OpenMP restatement of the reference host path"}
 
 
+def batched_bench(args, sp, torch, np, t, dims, ns, ng, plain_value):
+    """Batched form: --batch bands of the same 128^3 r=40 set through one
+    multi_transform_backward/forward call each, FP64 and FP32; band-pairs/s
+    and the FP32 / plain-FP64 ratio. Device-resident, values as above."""
+    from tests.workloads import complex_values
+    out = {"bands": args.batch}
+    for prec in ("double", "float"):
+        real = np.float64 if prec == "double" else np.float32
+        cls = sp.Transform if prec == "double" else sp.TransformFloat
+        trs, ins, outs = [], [], []
+        for k in range(args.batch):
+            tr = cls(-1, sp.SPFFT_PU_GPU, sp.SPFFT_TRANS_C2C, *dims, ns, sp.SPFFT_INDEX_TRIPLETS, t)
+            tr.set_execution_mode(sp.SPFFT_EXEC_ASYNCHRONOUS)
+            v = complex_values(0x5EED0100 + k, ns)
+            ins.append(torch.from_numpy(v.view(np.float64).astype(real)).cuda())
+            outs.append(torch.empty(2 * ns, dtype=ins[-1].dtype, device="cuda"))
+            trs.append(tr)
+        locs = [sp.SPFFT_PU_GPU] * args.batch
+        scal = [sp.SPFFT_FULL_SCALING] * args.batch
+
+        def step():
+            sp.multi_transform_backward(trs, ins, locs)
+            sp.multi_transform_forward(trs, locs, outs, scal)
+
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+        steps = max(5, args.steps // args.batch)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        rate = steps * args.batch / el
+        rb = np.dtype(real).itemsize
+        pair_bytes = 2 * (ns * (2 * rb + 4) + ng * 2 * rb)
+        out[prec] = {"band_pairs_per_s": rate, "ms_per_call_pair": el / steps * 1e3,
+                     "frac_hbm": rate * pair_bytes / 1e9 / HBM_PEAK_GBS}
+        log(f"batched {prec}: {rate:.1f} band-pairs/s ({args.batch} bands per call)")
+        del trs, ins, outs
+        torch.cuda.empty_cache()
+    out["fp32_vs_plain_fp64"] = out["float"]["band_pairs_per_s"] / plain_value
+    out["fp64_vs_plain_fp64"] = out["double"]["band_pairs_per_s"] / plain_value
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +141,8 @@ def main():
     ap.add_argument("--precision", choices=["double", "float"], default="double")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="transforms per multi_transform call in the batched section (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched "
@@ -215,6 +263,8 @@ def main():
         },
         "kernels": kernels,
     }
+    if args.batch > 0:
+        result["batched"] = batched_bench(args, sp, torch, np, t, dims, ns, ng, value)
     if not args.no_cpu_baseline:
         v0 = complex_values(0x5EED0002, ns)
         result["cpu_baseline"] = cpu_baseline(t, dims, v0, args.cpu_seconds)

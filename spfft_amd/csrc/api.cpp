@@ -60,9 +60,11 @@ std::shared_ptr<TransformInternal<T>> make_independent(int maxNumThreads,
 
 }  // namespace
 
-// Batched execution (reference multi_transform_internal.hpp:46-175): every
+// Batched execution (reference multi_transform_internal.hpp:46-175).
+// Transforms of one shape on the fixed-length path run as one launch chain
+// (one z launch and one xy launch for the whole batch); otherwise every
 // transform is enqueued on its own stream before any is synchronised, so
-// independent transforms overlap on the GPU.
+// independent transforms still overlap on the GPU.
 template <typename TransformType>
 class MultiTransformInternal {
 public:
@@ -71,6 +73,16 @@ public:
   static void forward(int n, TransformType* transforms, const ValueType* const* inputs,
                       ValueType* const* outputs, const SpfftScalingType* scalings) {
     check(n, transforms);
+    {
+      std::vector<TransformInternal<ValueType>*> ts;
+      for (int t = 0; t < n; ++t) ts.push_back(transforms[t].transform_.get());
+      if (TransformInternal<ValueType>::batch_forward(
+              ts, std::vector<const ValueType*>(inputs, inputs + n),
+              std::vector<ValueType*>(outputs, outputs + n),
+              std::vector<SpfftScalingType>(scalings, scalings + n))) {
+        return;
+      }
+    }
     for (int t = 0; t < n; ++t) {
       transforms[t].transform_->forward_enqueue(inputs[t], outputs[t], scalings[t]);
     }
@@ -89,6 +101,15 @@ public:
   static void backward(int n, TransformType* transforms, const ValueType* const* inputs,
                        ValueType* const* outputs) {
     check(n, transforms);
+    {
+      std::vector<TransformInternal<ValueType>*> ts;
+      for (int t = 0; t < n; ++t) ts.push_back(transforms[t].transform_.get());
+      if (TransformInternal<ValueType>::batch_backward(
+              ts, std::vector<const ValueType*>(inputs, inputs + n),
+              std::vector<ValueType*>(outputs, outputs + n))) {
+        return;
+      }
+    }
     for (int t = 0; t < n; ++t) transforms[t].transform_->backward_enqueue(inputs[t], outputs[t]);
     for (int t = 0; t < n; ++t) transforms[t].transform_->synchronize();
   }

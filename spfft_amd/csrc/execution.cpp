@@ -1,5 +1,6 @@
 #include "execution.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 #include <utility>
 #include <vector>
@@ -73,12 +74,25 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
           fast_path_supported<T>(map_->dimX, map_->dimY, map_->dimZ, numCols_);
   if (fast_) {
     const int ny = map_->dimY;
-    std::vector<int16_t> rel(static_cast<size_t>(numCols_) * ny, int16_t(-1));
+    std::vector<int> at(static_cast<size_t>(numCols_) * ny, -1);
     for (int s = 0; s < map_->num_sticks(); ++s) {
-      const int u = t.stickCol[s];
-      rel[static_cast<size_t>(u) * ny + t.stickY[s]] = static_cast<int16_t>(s - t.colBegin[u]);
+      at[static_cast<size_t>(t.stickCol[s]) * ny + t.stickY[s]] = s;
     }
-    stickRel_ = DeviceBuffer<int16_t>::from_host(rel);
+    stickAt_ = DeviceBuffer<int>::from_host(at);
+    // Entry list for the z kernels: value index and position in the LDS tile
+    // of the workgroup that owns the stick; earlier copies of a repeated
+    // triplet are flagged so that the last one wins on scatter.
+    int S = 0, LZ = 0, shift = 0;
+    fast_z_geometry<T>(map_->dimZ, &S, &LZ, &shift);
+    std::vector<int2> ent(map_->num_values());
+    for (int e = 0; e < map_->num_values(); ++e) {
+      const int slot = t.entrySlot[e];
+      const int st = slot / map_->dimZ, z = slot % map_->dimZ;
+      int pos = (st % S) * LZ + z + (z >> shift);
+      if (e + 1 < map_->num_values() && t.entrySlot[e + 1] == slot) pos |= int(0x80000000u);
+      ent[e] = make_int2(t.entryValue[e], pos);
+    }
+    zEntries_ = DeviceBuffer<int2>::from_host(ent);
     const int classes = fast_classes(map_->dimX);
     std::vector<int> resBegin(classes + 1, 0), resSticks;
     resSticks.reserve(map_->num_sticks());
@@ -89,6 +103,12 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
       }
     }
     resBegin[classes] = static_cast<int>(resSticks.size());
+    std::vector<int> resYM(resSticks.size());
+    for (size_t i = 0; i < resSticks.size(); ++i) {
+      const int st = resSticks[i];
+      resYM[i] = (t.stickY[st] << 16) | (t.colX[t.stickCol[st]] / classes);
+    }
+    resYM_ = DeviceBuffer<int>::from_host(resYM);
     std::vector<int> xCol(map_->dimX, -1);
     for (int u = 0; u < numCols_; ++u) xCol[t.colX[u]] = u;
     xCol_ = DeviceBuffer<int>::from_host(xCol);
@@ -168,10 +188,13 @@ FastArgs<T> ExecutionGPU<T>::fast_args() const {
   a.stickCol = stickCol_.data();
   a.colX = colX_.data();
   a.colBegin = colBegin_.data();
-  a.stickRel = stickRel_.data();
+  a.stickAt = stickAt_.data();
+  a.zEntries = zEntries_.data();
+  a.pitch = fast_pitch<T>(map_->num_sticks());
   a.resSticks = resSticks_.data();
   a.resBegin = resBegin_.data();
   a.xCol = xCol_.data();
+  a.resYM = resYM_.data();
   a.twX = twX_.data();
   a.twY = twY_.data();
   a.twZ = twZ_.data();
@@ -298,6 +321,62 @@ void ExecutionGPU<T>::synchronize(SpfftExecType mode, bool concurrent) {
   } else {
     check_hip(hipStreamSynchronize(stream_));
   }
+}
+
+template <typename T>
+bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
+                                const std::vector<const T*>& in, const std::vector<T*>& out,
+                                bool forward, const std::vector<SpfftScalingType>& scaling) {
+  const size_t n = ex.size();
+  if (n < 2) return false;
+  const IndexMap& m0 = *ex[0]->map_;
+  for (const ExecutionGPU<T>* e : ex) {
+    const IndexMap& m = *e->map_;
+    if (!e->fast_ || m.dimX != m0.dimX || m.dimY != m0.dimY || m.dimZ != m0.dimZ ||
+        m.type != m0.type || m.num_values() == 0 || e->grid_->device_id() != ex[0]->grid_->device_id()) {
+      return false;
+    }
+  }
+  int S = 0, LZ = 0, shift = 0;
+  fast_z_geometry<T>(m0.dimZ, &S, &LZ, &shift);
+  std::vector<FastArgs<T>> args(n);
+  std::vector<int2> zmap;
+  int maxNcol = 0;
+  for (size_t t = 0; t < n; ++t) {
+    auto i = translate_pointer(in[t]);
+    auto o = translate_pointer(out[t]);
+    if (!i.second || !o.second) return false;
+    FastArgs<T> f = ex[t]->fast_args();
+    f.in = i.second;
+    f.out = o.second;
+    f.useScale = forward && scaling[t] == SPFFT_FULL_SCALING ? 1 : 0;
+    f.pitchA = fast_xy_pitch_a<T>(m0.dimX, f.ncol);
+    maxNcol = std::max(maxNcol, f.ncol);
+    const int blocks = (f.nst + S - 1) / S;
+    for (int b = 0; b < blocks; ++b) zmap.push_back(make_int2(static_cast<int>(t), b));
+    args[t] = f;
+  }
+  if (hipGetLastError() != hipSuccess) throw GPUPrecedingError();
+  // Argument tables live in stream-ordered allocations on the default stream.
+  FastArgs<T>* dArgs = nullptr;
+  int2* dMap = nullptr;
+  check_hip(hipMallocAsync(reinterpret_cast<void**>(&dArgs), n * sizeof(FastArgs<T>), nullptr));
+  check_hip(hipMallocAsync(reinterpret_cast<void**>(&dMap), zmap.size() * sizeof(int2), nullptr));
+  check_hip(hipMemcpyAsync(dArgs, args.data(), n * sizeof(FastArgs<T>), hipMemcpyHostToDevice,
+                           nullptr));
+  check_hip(hipMemcpyAsync(dMap, zmap.data(), zmap.size() * sizeof(int2), hipMemcpyHostToDevice,
+                           nullptr));
+  const int blocks = static_cast<int>(zmap.size());
+  if (forward) {
+    launch_fast_xy_batch<T>(true, m0.dimX, dArgs, static_cast<int>(n), m0.dimZ, maxNcol, nullptr);
+    launch_fast_z_batch<T>(true, m0.dimZ, dArgs, dMap, blocks, nullptr);
+  } else {
+    launch_fast_z_batch<T>(false, m0.dimZ, dArgs, dMap, blocks, nullptr);
+    launch_fast_xy_batch<T>(false, m0.dimX, dArgs, static_cast<int>(n), m0.dimZ, maxNcol, nullptr);
+  }
+  check_hip(hipFreeAsync(dArgs, nullptr));
+  check_hip(hipFreeAsync(dMap, nullptr));
+  return true;
 }
 
 template class ExecutionGPU<double>;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <memory>
+#include <vector>
 
 #include "grid_internal.hpp"
 #include "index_map.hpp"
@@ -37,6 +38,14 @@ public:
   // additionally makes the default stream wait for its stream.
   void synchronize(SpfftExecType mode, bool concurrent = false);
 
+  // Batched execution of several transforms of one shape on the fixed-length
+  // path: one z launch and one xy launch for the whole batch, on the default
+  // stream. Returns false without doing anything when the batch does not
+  // qualify (different shapes, generic lengths, host pointers, empty sets).
+  static bool run_batch(const std::vector<ExecutionGPU<T>*>& ex, const std::vector<const T*>& in,
+                        const std::vector<T*>& out, bool forward,
+                        const std::vector<SpfftScalingType>& scaling);
+
   T* space_domain_gpu() const { return grid_->space_gpu(); }
   T* space_domain_host() const { return grid_->space_host(); }
   hipStream_t stream() const { return stream_; }
@@ -61,8 +70,9 @@ private:
   DeviceBuffer<int> entryValue_, entrySlot_, stickBegin_, stickY_, stickCol_, colX_, colBegin_;
   // fixed-length C2C path (two kernels per direction)
   bool fast_ = false;
-  DeviceBuffer<int16_t> stickRel_;
-  DeviceBuffer<int> resSticks_, resBegin_, xCol_;
+  DeviceBuffer<int> stickAt_;
+  DeviceBuffer<int2> zEntries_;
+  DeviceBuffer<int> resSticks_, resBegin_, xCol_, resYM_;
 };
 
 }  // namespace spfft

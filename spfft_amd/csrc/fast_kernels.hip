@@ -78,7 +78,7 @@ __device__ __forceinline__ void wave_sync() {
 
 // One Stockham pass (and, recursively, the remaining ones) of the wave-level
 // FFT. `line` is this line's exchange region: element i at line[phys(i) * es]
-// (es = 1) or line[i * es] (column regions, es > 1). The twiddle table holds
+// (rows: es = 1; columns of a tile: es = row length). The twiddle table holds
 // W_{N * TWS}. NS = product of the radices already applied.
 template <typename T, int N, bool BWD, int TWS, int NS = 1>
 __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
@@ -109,7 +109,7 @@ __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int i = o + r * NS;
-        line[(es == 1 ? phys<T>(i) : i * es)] = u[r];
+        line[phys<T>(i) * es] = u[r];
       }
     }
   }
@@ -118,7 +118,7 @@ __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = q + PL * e;
-      v[e] = line[(es == 1 ? phys<T>(i) : i * es)];
+      v[e] = line[phys<T>(i) * es];
     }
     wave_sync();
     wave_fft<T, N, BWD, TWS, NS * R>(v, line, es, q, tw);
@@ -141,27 +141,31 @@ struct ZShape {
   static constexpr int NT = 256;
   static constexpr int S = (NT / kWave) * Line<NZ>::PER_WAVE;  // sticks per workgroup
   static constexpr int LZ = padded_len<T>(NZ);
-  static size_t lds() { return static_cast<size_t>(S) * LZ * sizeof(Cx<T>); }
+  // one spare element after the tile absorbs repeated triplets
+  static size_t lds() { return (static_cast<size_t>(S) * LZ + 1) * sizeof(Cx<T>); }
 };
 
 template <typename T, int NZ>
-__global__ void __launch_bounds__((ZShape<T, NZ>::NT)) k_z_backward_fast(FastArgs<T> a) {
+__device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
   using Z = ZShape<T, NZ>;
   using L = Line<NZ>;
   constexpr int S = Z::S, LZ = Z::LZ, NT = Z::NT;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
-  const int s0 = blockIdx.x * S;
+  const int s0 = bid * S;
   const int ns = min(S, a.nst - s0);
   for (int t = threadIdx.x; t < S * LZ; t += NT) tile[t] = Cx<T>{T(0), T(0)};
   __syncthreads();
+  // Entries carry their tile position (a repeated triplet's earlier copies
+  // point at the spare slot after the tile, so the last one wins).
   const int e0 = a.stickBegin[s0], e1 = a.stickBegin[s0 + ns];
-  for (int e = e0 + threadIdx.x; e < e1; e += NT) {
-    const int slot = a.entrySlot[e];
-    if (e + 1 < e1 && a.entrySlot[e + 1] == slot) continue;  // repeated triplet: last one wins
-    const int rel = slot - s0 * NZ;
-    const int sl = rel / NZ;
-    const size_t v = static_cast<size_t>(a.entryValue[e]);
-    tile[sl * LZ + phys<T>(rel - sl * NZ)] = Cx<T>{a.in[2 * v], a.in[2 * v + 1]};
+  for (int e = e0 + threadIdx.x; e < e1; e += 2 * NT) {
+    const bool two = e + NT < e1;
+    const int2 ea = a.zEntries[e];
+    const int2 eb = a.zEntries[two ? e + NT : e];
+    const Cx<T> va = reinterpret_cast<const Cx<T>*>(a.in)[ea.x];
+    const Cx<T> vb = reinterpret_cast<const Cx<T>*>(a.in)[eb.x];
+    tile[ea.y < 0 ? S * LZ : ea.y] = va;
+    if (two) tile[eb.y < 0 ? S * LZ : eb.y] = vb;
   }
   __syncthreads();
   const int lane = threadIdx.x % kWave;
@@ -178,21 +182,21 @@ __global__ void __launch_bounds__((ZShape<T, NZ>::NT)) k_z_backward_fast(FastArg
   __syncthreads();
   for (int t = threadIdx.x; t < S * NZ; t += NT) {
     const int s = t % S, zz = t / S;
-    if (s < ns) a.zbuf[static_cast<size_t>(zz) * a.nst + s0 + s] = tile[s * LZ + phys<T>(zz)];
+    if (s < ns) a.zbuf[static_cast<size_t>(zz) * a.pitch + s0 + s] = tile[s * LZ + phys<T>(zz)];
   }
 }
 
 template <typename T, int NZ>
-__global__ void __launch_bounds__((ZShape<T, NZ>::NT)) k_z_forward_fast(FastArgs<T> a) {
+__device__ __forceinline__ void z_forward_body(const FastArgs<T>& a, int bid) {
   using Z = ZShape<T, NZ>;
   using L = Line<NZ>;
   constexpr int S = Z::S, LZ = Z::LZ, NT = Z::NT;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
-  const int s0 = blockIdx.x * S;
+  const int s0 = bid * S;
   const int ns = min(S, a.nst - s0);
   for (int t = threadIdx.x; t < S * NZ; t += NT) {
     const int s = t % S, zz = t / S;
-    tile[s * LZ + phys<T>(zz)] = s < ns ? a.zbuf[static_cast<size_t>(zz) * a.nst + s0 + s]
+    tile[s * LZ + phys<T>(zz)] = s < ns ? a.zbuf[static_cast<size_t>(zz) * a.pitch + s0 + s]
                                         : Cx<T>{T(0), T(0)};
   }
   __syncthreads();
@@ -210,13 +214,10 @@ __global__ void __launch_bounds__((ZShape<T, NZ>::NT)) k_z_forward_fast(FastArgs
   __syncthreads();
   const int e0 = a.stickBegin[s0], e1 = a.stickBegin[s0 + ns];
   for (int e = e0 + threadIdx.x; e < e1; e += NT) {
-    const int rel = a.entrySlot[e] - s0 * NZ;
-    const int s = rel / NZ;
-    Cx<T> c = tile[s * LZ + phys<T>(rel - s * NZ)];
+    const int2 en = a.zEntries[e];
+    Cx<T> c = tile[en.y & 0x7fffffff];
     if (a.useScale) c = cscale(c, a.scale);
-    const size_t v = static_cast<size_t>(a.entryValue[e]);
-    a.out[2 * v] = c.re;
-    a.out[2 * v + 1] = c.im;
+    reinterpret_cast<Cx<T>*>(a.out)[en.x] = c;
   }
 }
 
@@ -228,89 +229,111 @@ struct XYShape {
   static constexpr int M = N / R;               // rows (bwd) / columns (fwd) per class
   static constexpr int NT = N >= 128 && N < 192 ? 512 : 256;
   static constexpr int NW = NT / kWave;
-  // backward: A = [columns][LA] (y transforms), reused as B = [rows][LB] (x transforms)
-  static constexpr int LA = make_odd(padded_len<T>(M));
+  // backward: A = [m][LU] (column u at offset u, y transforms in column
+  // mode), reused as B = [rows][LB] (x transforms)
+  static constexpr int ROWS_A = padded_len<T>(M);
   static constexpr int LB = padded_len<T>(N);
   static constexpr int ROWS_ALLOC = round_up(M, Line<N>::PER_WAVE);
   static_assert(NW * Line<N>::PER_WAVE >= M, "all rows of a class in one round");
   // forward: Tb = [rows y][LT], columns transformed in place with stride LT;
   // the rows have room for the spare columns of a partial last wave
   static constexpr int LT = make_odd(padded_len<T>(round_up(M, Line<N>::PER_WAVE)));
+  // Row pitch of A: whole waves of columns, and == 4 (mod 16) so that the
+  // lanes of a 16-lane LDS phase hit distinct bank groups.
+  static int pitch_a(int ncol) { return round_up(round_up(ncol, Line<M>::PER_WAVE) - 4, 16) + 4; }
   static size_t lds_backward(int ncol) {
-    const size_t a = static_cast<size_t>(round_up(ncol, Line<M>::PER_WAVE)) * LA;
+    const size_t a = static_cast<size_t>(ROWS_A) * pitch_a(ncol);
     const size_t b = static_cast<size_t>(ROWS_ALLOC) * LB;
     return (a > b ? a : b) * sizeof(Cx<T>);
   }
-  static size_t lds_forward() {
-    return static_cast<size_t>(round_up(N, Line<M>::PER_WAVE)) * LT * sizeof(Cx<T>);
-  }
+  // rows of the tile: the y lines' exchange (padded index map) reaches row
+  // padded_len(N) - 1; the x pass may round the row count up to whole waves
+  static constexpr int ROWS_FWD =
+      padded_len<T>(N) > round_up(N, Line<M>::PER_WAVE) ? padded_len<T>(N)
+                                                       : round_up(N, Line<M>::PER_WAVE);
+  static size_t lds_forward() { return static_cast<size_t>(ROWS_FWD) * LT * sizeof(Cx<T>); }
 };
 
 template <typename T, int N>
-__global__ void __launch_bounds__((XYShape<T, N>::NT)) k_xy_backward_fast(FastArgs<T> a) {
+__device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) {
   using S = XYShape<T, N>;
   using LM = Line<S::M>;
   using LX = Line<N>;
-  constexpr int R = S::R, M = S::M, LA = S::LA, LB = S::LB, NW = S::NW;
+  constexpr int R = S::R, M = S::M, LB = S::LB, NW = S::NW;
   const int ncol = a.ncol;
+  const int LU = a.pitchA;
   Cx<T>* A = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
   int z, r;
-  plane_class<R>(blockIdx.x, z, r);
+  plane_class<R>(bid, z, r);
   if (z >= a.nz) return;
-  const Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.nst;
+  const Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  // column of each x this lane handles in the x transform (loaded early)
+  int uu[LX::E];
+#pragma unroll
+  for (int e = 0; e < LX::E; ++e) uu[e] = a.xCol[lane % LX::P + LX::P * e];
 
   // y: for each column u, decimation in frequency to the rows y = r + R m
-  // (t[j] = W^(r j) sum_l X[j + M l] W_R^(r l)), then the M-point FFT.
+  // (t[j] = W^(r j) sum_l X[j + M l] W_R^(r l)), then the M-point FFT. The
+  // gathers are branch free (absent sticks read entry 0 and are masked), so
+  // a lane's loads are all in flight together.
   {
     const int q = lane % LM::P;
+    Cx<T> wl[R];
+    wl[0] = Cx<T>{T(1), T(0)};
+#pragma unroll
+    for (int l = 1; l < R; ++l) wl[l] = twiddle<true>(a.twY, ((r * l) % R) * M);
     for (int u0 = wave * LM::PER_WAVE; u0 < ncol; u0 += NW * LM::PER_WAVE) {
       const int u = u0 + lane / LM::P;
+      const int uc = u < ncol ? u : ncol - 1;
+      const int* rel = a.stickAt + static_cast<size_t>(uc) * N;
+      const Cx<T>* col = zrow;
       Cx<T> v[LM::E];
-      if (u < ncol) {
-        const int16_t* rel = a.stickRel + static_cast<size_t>(u) * N;
-        const Cx<T>* col = zrow + a.colBegin[u];
-        int s[LM::E][R];
+      constexpr int EC = LM::E % 4 == 0 ? 4 : LM::E;  // elements gathered per batch
 #pragma unroll
-        for (int e = 0; e < LM::E; ++e) {
+      for (int e0 = 0; e0 < LM::E; e0 += EC) {
+        int sv[EC][R];
 #pragma unroll
-          for (int l = 0; l < R; ++l) s[e][l] = rel[q + LM::P * e + M * l];
+        for (int e = 0; e < EC; ++e) {
+#pragma unroll
+          for (int l = 0; l < R; ++l) sv[e][l] = rel[q + LM::P * (e0 + e) + M * l];
+        }
+        Cx<T> xv[EC][R];
+#pragma unroll
+        for (int e = 0; e < EC; ++e) {
+#pragma unroll
+          for (int l = 0; l < R; ++l) xv[e][l] = col[sv[e][l] >= 0 ? sv[e][l] : 0];
         }
 #pragma unroll
-        for (int e = 0; e < LM::E; ++e) {
-          const int j = q + LM::P * e;
+        for (int e = 0; e < EC; ++e) {
+          const int j = q + LM::P * (e0 + e);
+          const Cx<T> wj = twiddle<true>(a.twY, r * j);
           Cx<T> acc = {T(0), T(0)};
 #pragma unroll
           for (int l = 0; l < R; ++l) {
-            if (s[e][l] >= 0) {
-              Cx<T> x = col[s[e][l]];
-              const int ex = ((r * l) % R) * M;
-              if (ex) x = cmul(x, twiddle<true>(a.twY, ex));
-              acc = cadd(acc, x);
-            }
+            const Cx<T> t = l ? cmul(xv[e][l], wl[l]) : xv[e][l];
+            if (sv[e][l] >= 0) acc = cadd(acc, t);
           }
-          v[e] = r ? cmul(acc, twiddle<true>(a.twY, r * j)) : acc;
+          v[e0 + e] = (u < ncol) ? cmul(acc, wj) : Cx<T>{T(0), T(0)};
         }
-      } else {
-#pragma unroll
-        for (int e = 0; e < LM::E; ++e) v[e] = Cx<T>{T(0), T(0)};
       }
-      Cx<T>* line = A + u * LA;
-      wave_fft<T, M, true, R>(v, line, 1, q, a.twY);
+      Cx<T>* tcol = A + u;
+      wave_fft<T, M, true, R>(v, tcol, LU, q, a.twY);
 #pragma unroll
-      for (int e = 0; e < LM::E; ++e) line[phys<T>(q + LM::P * e)] = v[e];
+      for (int e = 0; e < LM::E; ++e) tcol[(q + LM::P * e) * LU] = v[e];
     }
   }
   __syncthreads();
 
   // x: row m (y = r + R m) holds column u's value at x = colX[u].
   const int m = wave * LX::PER_WAVE + lane / LX::P;
+  const int mc = m < M ? m : M - 1;
   const int q = lane % LX::P;
   Cx<T> v[LX::E];
 #pragma unroll
   for (int e = 0; e < LX::E; ++e) {
-    const int u = a.xCol[q + LX::P * e];
-    v[e] = (m < M && u >= 0) ? A[u * LA + phys<T>(m)] : Cx<T>{T(0), T(0)};
+    const Cx<T> x = A[mc * LU + (uu[e] >= 0 ? uu[e] : 0)];
+    v[e] = (m < M && uu[e] >= 0) ? x : Cx<T>{T(0), T(0)};
   }
   __syncthreads();  // A is dead from here on; its space holds the row lines
   if (wave * LX::PER_WAVE < M) {
@@ -326,44 +349,51 @@ __global__ void __launch_bounds__((XYShape<T, N>::NT)) k_xy_backward_fast(FastAr
 }
 
 template <typename T, int N>
-__global__ void __launch_bounds__((XYShape<T, N>::NT)) k_xy_forward_fast(FastArgs<T> a) {
+__device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
   using S = XYShape<T, N>;
   using LM = Line<S::M>;
   using LY = Line<N>;
   constexpr int R = S::R, MX = S::M, LT = S::LT, NW = S::NW;
   Cx<T>* Tb = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
   int z, c;
-  plane_class<R>(blockIdx.x, z, c);
+  plane_class<R>(bid, z, c);
   if (z >= a.nz) return;
   const Cx<T>* in = reinterpret_cast<const Cx<T>*>(a.in) + static_cast<size_t>(z) * N * N;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
 
   // x: every row, decimation in frequency to the columns x = c + R m, then
-  // the MX-point FFT; row y's result lands in Tb[y][m].
+  // the MX-point FFT; row y's result lands in Tb[y][m]. Class twiddles are
+  // loaded once; the row loads of a lane are issued together.
   {
     const int q = lane % LM::P;
+    Cx<T> wl[R];
+    wl[0] = Cx<T>{T(1), T(0)};
+#pragma unroll
+    for (int l = 1; l < R; ++l) wl[l] = twiddle<false>(a.twX, ((c * l) % R) * MX);
+    Cx<T> wj[LM::E];
+#pragma unroll
+    for (int e = 0; e < LM::E; ++e) wj[e] = twiddle<false>(a.twX, c * (q + LM::P * e));
     for (int y0 = wave * LM::PER_WAVE; y0 < N; y0 += NW * LM::PER_WAVE) {
       const int y = y0 + lane / LM::P;
+      const int yc = y < N ? y : N - 1;
+      const Cx<T>* row = in + static_cast<size_t>(yc) * N;
       Cx<T> v[LM::E];
-      if (y < N) {
-        const Cx<T>* row = in + static_cast<size_t>(y) * N;
+      constexpr int EC = LM::E % 4 == 0 ? 4 : LM::E;
 #pragma unroll
-        for (int e = 0; e < LM::E; ++e) {
-          const int j = q + LM::P * e;
-          Cx<T> x[R];
+      for (int e0 = 0; e0 < LM::E; e0 += EC) {
+        Cx<T> xv[EC][R];
 #pragma unroll
-          for (int l = 0; l < R; ++l) x[l] = row[j + MX * l];
-          Cx<T> acc = x[0];
+        for (int e = 0; e < EC; ++e) {
 #pragma unroll
-          for (int l = 1; l < R; ++l) {
-            const int ex = ((c * l) % R) * MX;
-            acc = cadd(acc, ex ? cmul(x[l], twiddle<false>(a.twX, ex)) : x[l]);
-          }
-          v[e] = c ? cmul(acc, twiddle<false>(a.twX, c * j)) : acc;
+          for (int l = 0; l < R; ++l) xv[e][l] = row[q + LM::P * (e0 + e) + MX * l];
         }
-      } else {
 #pragma unroll
-        for (int e = 0; e < LM::E; ++e) v[e] = Cx<T>{T(0), T(0)};
+        for (int e = 0; e < EC; ++e) {
+          Cx<T> acc = xv[e][0];
+#pragma unroll
+          for (int l = 1; l < R; ++l) acc = cadd(acc, cmul(xv[e][l], wl[l]));
+          v[e0 + e] = cmul(acc, wj[e0 + e]);
+        }
       }
       Cx<T>* line = Tb + y * LT;
       wave_fft<T, MX, false, R>(v, line, 1, q, a.twX);
@@ -393,12 +423,58 @@ __global__ void __launch_bounds__((XYShape<T, N>::NT)) k_xy_forward_fast(FastArg
   }
   __syncthreads();
   const int i0 = a.resBegin[c], i1 = a.resBegin[c + 1];
-  Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.nst;
+  Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
   for (int i = i0 + threadIdx.x; i < i1; i += S::NT) {
     const int s = a.resSticks[i];
-    const int m = a.colX[a.stickCol[s]] / R;
-    zrow[s] = Tb[a.stickY[s] * LT + phys<T>(m)];
+    const int ym = a.resYM[i];  // y << 16 | column m within the class
+    zrow[s] = Tb[(ym >> 16) * LT + phys<T>(ym & 0xffff)];
   }
+}
+
+// ---- entry points: one transform (arguments by value) or a batch of
+// transforms of one shape (argument table in device memory). A batched z
+// launch maps each workgroup to (transform, block) through `map`; a batched
+// xy launch gives every transform the same number of workgroups.
+
+template <typename T, int NZ>
+__global__ void __launch_bounds__((ZShape<T, NZ>::NT)) k_z_backward_fast(FastArgs<T> a) {
+  z_backward_body<T, NZ>(a, blockIdx.x);
+}
+template <typename T, int NZ>
+__global__ void __launch_bounds__((ZShape<T, NZ>::NT)) k_z_forward_fast(FastArgs<T> a) {
+  z_forward_body<T, NZ>(a, blockIdx.x);
+}
+template <typename T, int N>
+__global__ void __launch_bounds__((XYShape<T, N>::NT)) k_xy_backward_fast(FastArgs<T> a) {
+  xy_backward_body<T, N>(a, blockIdx.x);
+}
+template <typename T, int N>
+__global__ void __launch_bounds__((XYShape<T, N>::NT)) k_xy_forward_fast(FastArgs<T> a) {
+  xy_forward_body<T, N>(a, blockIdx.x);
+}
+template <typename T, int NZ>
+__global__ void __launch_bounds__((ZShape<T, NZ>::NT))
+    k_z_backward_batch(const FastArgs<T>* __restrict__ batch, const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  z_backward_body<T, NZ>(batch[m.x], m.y);
+}
+template <typename T, int NZ>
+__global__ void __launch_bounds__((ZShape<T, NZ>::NT))
+    k_z_forward_batch(const FastArgs<T>* __restrict__ batch, const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  z_forward_body<T, NZ>(batch[m.x], m.y);
+}
+template <typename T, int N>
+__global__ void __launch_bounds__((XYShape<T, N>::NT))
+    k_xy_backward_batch(const FastArgs<T>* __restrict__ batch, int perTransform) {
+  const int t = blockIdx.x / perTransform;
+  xy_backward_body<T, N>(batch[t], blockIdx.x - t * perTransform);
+}
+template <typename T, int N>
+__global__ void __launch_bounds__((XYShape<T, N>::NT))
+    k_xy_forward_batch(const FastArgs<T>* __restrict__ batch, int perTransform) {
+  const int t = blockIdx.x / perTransform;
+  xy_forward_body<T, N>(batch[t], blockIdx.x - t * perTransform);
 }
 
 // Raise the dynamic LDS limit of a kernel once per process (the attribute
@@ -444,9 +520,10 @@ int plane_grid(int nz) {
 }
 
 template <typename T, int N>
-void xy_backward_n(const FastArgs<T>& a, hipStream_t stream) {
+void xy_backward_n(FastArgs<T> a, hipStream_t stream) {
   using S = XYShape<T, N>;
   const size_t lds = S::lds_backward(a.ncol);
+  a.pitchA = S::pitch_a(a.ncol);
   auto kernel = k_xy_backward_fast<T, N>;
   prepare_lds(kernel, lds);
   prof::KernelSpan span(stream, "k_xy_backward_fast");
@@ -483,6 +560,59 @@ bool fast_length(int n) { return n == 32 || n == 64 || n == 96 || n == 128 || n 
   }
 
 }  // namespace
+
+template <typename T, int N>
+void xy_backward_batch_n(const FastArgs<T>* batch, int ntrans, int nz, int maxNcol,
+                         hipStream_t stream) {
+  using S = XYShape<T, N>;
+  const size_t lds = S::lds_backward(maxNcol);
+  auto kernel = k_xy_backward_batch<T, N>;
+  prepare_lds(kernel, lds);
+  prof::KernelSpan span(stream, "k_xy_backward_batch");
+  const int per = plane_grid<N>(nz);
+  hipLaunchKernelGGL(kernel, dim3(per * ntrans), dim3(S::NT), lds, stream, batch, per);
+  check_launch();
+}
+
+template <typename T, int N>
+void xy_forward_batch_n(const FastArgs<T>* batch, int ntrans, int nz, hipStream_t stream) {
+  using S = XYShape<T, N>;
+  const size_t lds = S::lds_forward();
+  auto kernel = k_xy_forward_batch<T, N>;
+  prepare_lds(kernel, lds);
+  prof::KernelSpan span(stream, "k_xy_forward_batch");
+  const int per = plane_grid<N>(nz);
+  hipLaunchKernelGGL(kernel, dim3(per * ntrans), dim3(S::NT), lds, stream, batch, per);
+  check_launch();
+}
+
+template <typename T, int NZ>
+void z_batch_n(bool forward, const FastArgs<T>* batch, const int2* map, int blocks,
+               hipStream_t stream) {
+  using Z = ZShape<T, NZ>;
+  if (forward) {
+    auto kernel = k_z_forward_batch<T, NZ>;
+    prepare_lds(kernel, Z::lds());
+    prof::KernelSpan span(stream, "k_z_forward_batch");
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(Z::NT), Z::lds(), stream, batch, map);
+  } else {
+    auto kernel = k_z_backward_batch<T, NZ>;
+    prepare_lds(kernel, Z::lds());
+    prof::KernelSpan span(stream, "k_z_backward_batch");
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(Z::NT), Z::lds(), stream, batch, map);
+  }
+  check_launch();
+}
+
+template <typename T>
+void fast_z_geometry(int nz, int* sticksPerBlock, int* lineLength, int* padShift) {
+#define SPFFT_GEOM(N)                          \
+  *sticksPerBlock = ZShape<T, N>::S;           \
+  *lineLength = ZShape<T, N>::LZ;              \
+  *padShift = pad_shift<T>()
+  SPFFT_FAST_SWITCH(nz, SPFFT_GEOM)
+#undef SPFFT_GEOM
+}
 
 template <typename T>
 bool fast_path_supported(int nx, int ny, int nz, int ncol) {
@@ -524,8 +654,45 @@ void launch_fast_xy_forward(int nx, int ny, const FastArgs<T>& a, hipStream_t st
 #undef SPFFT_CALL
 }
 
+template <typename T>
+int fast_xy_pitch_a(int n, int ncol) {
+  int p = 0;
+#define SPFFT_PITCH(N) p = XYShape<T, N>::pitch_a(ncol)
+  SPFFT_FAST_SWITCH(n, SPFFT_PITCH)
+#undef SPFFT_PITCH
+  return p;
+}
+
+template <typename T>
+void launch_fast_z_batch(bool forward, int nz, const FastArgs<T>* batch, const int2* map,
+                         int blocks, hipStream_t stream) {
+#define SPFFT_CALL(N) z_batch_n<T, N>(forward, batch, map, blocks, stream)
+  SPFFT_FAST_SWITCH(nz, SPFFT_CALL)
+#undef SPFFT_CALL
+}
+
+template <typename T>
+void launch_fast_xy_batch(bool forward, int n, const FastArgs<T>* batch, int ntrans, int nz,
+                          int maxNcol, hipStream_t stream) {
+  if (forward) {
+#define SPFFT_CALL(N) xy_forward_batch_n<T, N>(batch, ntrans, nz, stream)
+    SPFFT_FAST_SWITCH(n, SPFFT_CALL)
+#undef SPFFT_CALL
+  } else {
+#define SPFFT_CALL(N) xy_backward_batch_n<T, N>(batch, ntrans, nz, maxNcol, stream)
+    SPFFT_FAST_SWITCH(n, SPFFT_CALL)
+#undef SPFFT_CALL
+  }
+}
+
 #define SPFFT_INSTANTIATE_FAST(T)                                                        \
+  template int fast_xy_pitch_a<T>(int, int);                                             \
+  template void launch_fast_z_batch<T>(bool, int, const FastArgs<T>*, const int2*, int,  \
+                                       hipStream_t);                                     \
+  template void launch_fast_xy_batch<T>(bool, int, const FastArgs<T>*, int, int, int,    \
+                                        hipStream_t);                                    \
   template bool fast_path_supported<T>(int, int, int, int);                              \
+  template void fast_z_geometry<T>(int, int*, int*, int*);                               \
   template void launch_fast_z_backward<T>(int, const FastArgs<T>&, hipStream_t);         \
   template void launch_fast_z_forward<T>(int, const FastArgs<T>&, hipStream_t);          \
   template void launch_fast_xy_backward<T>(int, int, const FastArgs<T>&, hipStream_t);   \

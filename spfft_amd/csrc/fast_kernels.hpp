@@ -27,6 +27,8 @@ namespace spfft {
 template <typename T>
 struct FastArgs {
   int nz = 0, nst = 0, ncol = 0;
+  int pitch = 0;   // row pitch of zbuf in elements (nst rounded up to 128 bytes)
+  int pitchA = 0;  // row pitch of the backward xy kernel's column tile (set at launch)
   Cx<T>* zbuf = nullptr;
   const T* in = nullptr;
   T* out = nullptr;
@@ -37,10 +39,13 @@ struct FastArgs {
   const int* stickCol = nullptr;
   const int* colX = nullptr;
   const int* colBegin = nullptr;
-  const int16_t* stickRel = nullptr;  // [ncol][NY]: stick index within its column, or -1
+  const int* stickAt = nullptr;       // [ncol][NY]: stick index at (column, y), or -1
+  const int2* zEntries = nullptr;     // per value (sorted by slot): {value index,
+                                      //  LDS tile position | repeated << 31}
   const int* xCol = nullptr;          // [NX]: column id of x, or -1
   const int* resSticks = nullptr;     // sticks grouped by colX mod R
   const int* resBegin = nullptr;      // R + 1 offsets into resSticks
+  const int* resYM = nullptr;         // per resSticks entry: y << 16 | (x / R)
   const Cx<T>* twX = nullptr;
   const Cx<T>* twY = nullptr;
   const Cx<T>* twZ = nullptr;
@@ -51,9 +56,34 @@ struct FastArgs {
 // Residue classes per plane used by the xy kernels of length n.
 constexpr int fast_classes(int n) { return n >= 192 ? 8 : 4; }
 
+// Geometry of the z kernels' LDS tile: sticks per workgroup, padded line
+// length and the padding shift of the index map i -> i + (i >> shift).
+template <typename T>
+void fast_z_geometry(int nz, int* sticksPerBlock, int* lineLength, int* padShift);
+
+// Row pitch of the [z][stick] buffer: nst rounded up to a 128-byte multiple.
+template <typename T>
+inline int fast_pitch(int nst) {
+  const int q = static_cast<int>(128 / (2 * sizeof(T)));
+  return (nst + q - 1) / q * q;
+}
+
 // True when (nx, ny, nz) have fast kernels and their LDS fits for ncol columns.
 template <typename T>
 bool fast_path_supported(int nx, int ny, int nz, int ncol);
+
+// Row pitch of the backward xy kernel's column tile for ncol columns.
+template <typename T>
+int fast_xy_pitch_a(int n, int ncol);
+
+// Batched launches over transforms of one shape. `batch` is a device table of
+// arguments; `map` gives (transform, block) for every z workgroup.
+template <typename T>
+void launch_fast_z_batch(bool forward, int nz, const FastArgs<T>* batch, const int2* map,
+                         int blocks, hipStream_t stream);
+template <typename T>
+void launch_fast_xy_batch(bool forward, int n, const FastArgs<T>* batch, int ntrans, int nz,
+                          int maxNcol, hipStream_t stream);
 
 template <typename T>
 void launch_fast_z_backward(int nz, const FastArgs<T>& a, hipStream_t stream);

@@ -5,7 +5,7 @@
 // Device memory of a GPU grid, for maxima (X, Y, Z) and C columns:
 //   space : X*Y*Z complex  - space-domain data (real for R2C uses the front half)
 //   work  : X*Y*Z complex  - y-stage intermediate [z][y][column] and host staging
-//   sticks: Z*C complex    - z-stage intermediate [z][stick]
+//   sticks: Z*C' complex   - z-stage intermediate [z][stick], C' = C rounded up to 16
 // The pinned host mirror of the space domain is allocated on first use.
 #pragma once
 

@@ -88,6 +88,44 @@ void TransformInternal<T>::synchronize() {
   exec_->synchronize(execMode_, true);
 }
 
+namespace {
+template <typename T>
+bool run_batched(const std::vector<TransformInternal<T>*>& ts,
+                 const std::vector<std::shared_ptr<ExecutionGPU<T>>>& execs,
+                 const std::vector<const T*>& in, const std::vector<T*>& out, bool forward,
+                 const std::vector<SpfftScalingType>& scaling, int device) {
+  std::vector<ExecutionGPU<T>*> ex;
+  for (const auto& e : execs) ex.push_back(e.get());
+  DeviceGuard guard(device);
+  if (!ExecutionGPU<T>::run_batch(ex, in, out, forward, scaling)) return false;
+  bool sync = false;
+  for (const TransformInternal<T>* t : ts) sync |= t->execution_mode() == SPFFT_EXEC_SYNCHRONOUS;
+  if (sync) check_hip(hipStreamSynchronize(nullptr));
+  return true;
+}
+}  // namespace
+
+template <typename T>
+bool TransformInternal<T>::batch_backward(const std::vector<TransformInternal*>& ts,
+                                          const std::vector<const T*>& in,
+                                          const std::vector<T*>& out) {
+  if (ts.empty()) return false;
+  std::vector<std::shared_ptr<ExecutionGPU<T>>> execs;
+  for (TransformInternal* t : ts) execs.push_back(t->exec_);
+  return run_batched<T>(ts, execs, in, out, false, {}, ts[0]->device_id());
+}
+
+template <typename T>
+bool TransformInternal<T>::batch_forward(const std::vector<TransformInternal*>& ts,
+                                         const std::vector<const T*>& in,
+                                         const std::vector<T*>& out,
+                                         const std::vector<SpfftScalingType>& scaling) {
+  if (ts.empty()) return false;
+  std::vector<std::shared_ptr<ExecutionGPU<T>>> execs;
+  for (TransformInternal* t : ts) execs.push_back(t->exec_);
+  return run_batched<T>(ts, execs, in, out, true, scaling, ts[0]->device_id());
+}
+
 template class TransformInternal<double>;
 template class TransformInternal<float>;
 

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <memory>
+#include <vector>
 
 #include "execution.hpp"
 #include "grid_internal.hpp"
@@ -30,6 +31,14 @@ public:
   void forward_enqueue(const T* input, T* output, SpfftScalingType scaling);
   void backward_enqueue(const T* input, T* output);
   void synchronize();
+
+  // One fused launch chain for a batch of transforms of one shape (see
+  // ExecutionGPU::run_batch); false when the batch does not qualify.
+  static bool batch_backward(const std::vector<TransformInternal*>& ts,
+                             const std::vector<const T*>& in, const std::vector<T*>& out);
+  static bool batch_forward(const std::vector<TransformInternal*>& ts,
+                            const std::vector<const T*>& in, const std::vector<T*>& out,
+                            const std::vector<SpfftScalingType>& scaling);
 
   T* space_domain_data(SpfftProcessingUnitType location);
 

@@ -291,3 +291,42 @@ def test_fixed_length_path_random_sets(n, fill, precision):
     assert orc.rel_l2(gpu_backward(tr, v, dims, False, precision), want) <= tol
     f = gpu_forward(tr, want, False, sp.SPFFT_FULL_SCALING, precision)
     assert orc.rel_l2(f, o.forward(want, True)) <= tol
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("n", [64, 96, 128])
+def test_batched_multi_transform_uneven_sets(n, precision):
+    """k-point style batch: every transform has its own shifted sphere (uneven
+    sets), one fused launch chain per direction; each result matches the
+    oracle and the single-transform call."""
+    dims = (n, n, n)
+    rng_k = uniform(80 + n, 3 * 6).reshape(6, 3) * 0.5
+    radii = 0.22 * n + 0.08 * n * uniform01_like(81 + n, 6)
+    trs, sets, vals = [], [], []
+    for k in range(6):
+        t = sphere_triplets(float(radii[k]), kshift=tuple(rng_k[k]))
+        trs.append(build(dims, t, False, precision))
+        sets.append(t)
+        vals.append(complex_values(90 + k, len(t)))
+    dt = torch.float64 if precision == "double" else torch.float32
+    rd = rdt(precision)
+    ins = [dev(v.view(np.float64).astype(rd)) for v in vals]
+    spaces = [torch.empty(2 * n ** 3, dtype=dt, device="cuda") for _ in trs]
+    outs = [torch.empty(2 * len(t), dtype=dt, device="cuda") for t in sets]
+    sp.multi_transform_backward(trs, ins, spaces)
+    sp.multi_transform_forward(trs, spaces, outs, [sp.SPFFT_FULL_SCALING] * len(trs))
+    tol = FP64_TOL if precision == "double" else FP32_TOL
+    for k, t in enumerate(sets):
+        o = orc.OraclePlan(False, dims, t)
+        want = o.backward(vals[k])
+        got = spaces[k].cpu().numpy().astype(np.float64).view(np.complex128).reshape(n, n, n)
+        assert orc.rel_l2(got, want) <= tol, k
+        single = gpu_backward(trs[k], vals[k], dims, False, precision)
+        assert orc.rel_l2(got, single) <= (1e-15 if precision == "double" else 1e-7)
+        f = outs[k].cpu().numpy().astype(np.float64).view(np.complex128)
+        assert orc.rel_l2(f, o.forward(want, True)) <= tol, k
+
+
+def uniform01_like(seed, n):
+    from tests.workloads import uniform01
+    return uniform01(seed, n)
