@@ -69,9 +69,10 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
   yBatch_ = stage_batch<T>(map_->dimY, std::max(1, numCols_), &lds);
   xBatch_ = stage_batch<T>(map_->dimX, map_->dimY, &lds);
 
-  // Fixed-length C2C kernels when the lengths have compile-time schedules.
-  fast_ = map_->type == SPFFT_TRANS_C2C && !disable_fast_path() &&
-          fast_path_supported<T>(map_->dimX, map_->dimY, map_->dimZ, numCols_);
+  // Fixed-length kernels when the lengths have compile-time schedules.
+  fast_ = !disable_fast_path() &&
+          fast_path_supported<T>(map_->dimX, map_->dimY, map_->dimZ, numCols_,
+                                 map_->type == SPFFT_TRANS_R2C);
   if (fast_) {
     const int ny = map_->dimY;
     std::vector<int> at(static_cast<size_t>(numCols_) * ny, -1);
@@ -191,6 +192,8 @@ FastArgs<T> ExecutionGPU<T>::fast_args() const {
   a.stickAt = stickAt_.data();
   a.zEntries = zEntries_.data();
   a.pitch = fast_pitch<T>(map_->num_sticks());
+  a.r2c = map_->type == SPFFT_TRANS_R2C ? 1 : 0;
+  a.zeroZeroStick = map_->zeroZeroStick;
   a.resSticks = resSticks_.data();
   a.resBegin = resBegin_.data();
   a.xCol = xCol_.data();
@@ -367,12 +370,14 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
   check_hip(hipMemcpyAsync(dMap, zmap.data(), zmap.size() * sizeof(int2), hipMemcpyHostToDevice,
                            nullptr));
   const int blocks = static_cast<int>(zmap.size());
+  const bool r2c = m0.type == SPFFT_TRANS_R2C;
+  const int nt = static_cast<int>(n);
   if (forward) {
-    launch_fast_xy_batch<T>(true, m0.dimX, dArgs, static_cast<int>(n), m0.dimZ, maxNcol, nullptr);
-    launch_fast_z_batch<T>(true, m0.dimZ, dArgs, dMap, blocks, nullptr);
+    launch_fast_xy_batch<T>(true, r2c, m0.dimX, dArgs, nt, m0.dimZ, maxNcol, nullptr);
+    launch_fast_z_batch<T>(true, r2c, m0.dimZ, dArgs, dMap, blocks, nullptr);
   } else {
-    launch_fast_z_batch<T>(false, m0.dimZ, dArgs, dMap, blocks, nullptr);
-    launch_fast_xy_batch<T>(false, m0.dimX, dArgs, static_cast<int>(n), m0.dimZ, maxNcol, nullptr);
+    launch_fast_z_batch<T>(false, r2c, m0.dimZ, dArgs, dMap, blocks, nullptr);
+    launch_fast_xy_batch<T>(false, r2c, m0.dimX, dArgs, nt, m0.dimZ, maxNcol, nullptr);
   }
   check_hip(hipFreeAsync(dArgs, nullptr));
   check_hip(hipFreeAsync(dMap, nullptr));

@@ -145,7 +145,7 @@ struct ZShape {
   static size_t lds() { return (static_cast<size_t>(S) * LZ + 1) * sizeof(Cx<T>); }
 };
 
-template <typename T, int NZ>
+template <typename T, int NZ, bool RC>
 __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
   using Z = ZShape<T, NZ>;
   using L = Line<NZ>;
@@ -168,6 +168,23 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
     if (two) tile[eb.y < 0 ? S * LZ : eb.y] = vb;
   }
   __syncthreads();
+  if (RC && s0 == 0 && a.zeroZeroStick == 0) {
+    // Hermitian fill of the (0,0) stick (reference symmetry_host.hpp:72-89):
+    // a non-zero lower entry wins, an upper entry fills a zero lower one, the
+    // self-paired entry is conjugated.
+    for (int k = 1 + threadIdx.x; 2 * k <= NZ; k += NT) {
+      Cx<T>& lo = tile[phys<T>(k)];
+      Cx<T>& hi = tile[phys<T>(NZ - k)];
+      if (2 * k == NZ) {
+        if (cnonzero(lo)) lo = cconj(lo);
+      } else if (cnonzero(lo)) {
+        hi = cconj(lo);
+      } else if (cnonzero(hi)) {
+        lo = cconj(hi);
+      }
+    }
+    __syncthreads();
+  }
   const int lane = threadIdx.x % kWave;
   const int sl = (threadIdx.x / kWave) * L::PER_WAVE + lane / L::P;
   const int q = lane % L::P;
@@ -241,10 +258,14 @@ struct XYShape {
   // Row pitch of A: whole waves of columns, and == 4 (mod 16) so that the
   // lanes of a 16-lane LDS phase hit distinct bank groups.
   static int pitch_a(int ncol) { return round_up(round_up(ncol, Line<M>::PER_WAVE) - 4, 16) + 4; }
-  static size_t lds_backward(int ncol) {
+  // elements of the A/B region; R2C appends the filled x = 0 column (N)
+  static size_t tile_backward(int ncol) {
     const size_t a = static_cast<size_t>(ROWS_A) * pitch_a(ncol);
     const size_t b = static_cast<size_t>(ROWS_ALLOC) * LB;
-    return (a > b ? a : b) * sizeof(Cx<T>);
+    return a > b ? a : b;
+  }
+  static size_t lds_backward(int ncol, bool rc) {
+    return (tile_backward(ncol) + (rc ? N : 0)) * sizeof(Cx<T>);
   }
   // rows of the tile: the y lines' exchange (padded index map) reaches row
   // padded_len(N) - 1; the x pass may round the row count up to whole waves
@@ -254,7 +275,7 @@ struct XYShape {
   static size_t lds_forward() { return static_cast<size_t>(ROWS_FWD) * LT * sizeof(Cx<T>); }
 };
 
-template <typename T, int N>
+template <typename T, int N, bool RC>
 __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) {
   using S = XYShape<T, N>;
   using LM = Line<S::M>;
@@ -268,10 +289,40 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
   if (z >= a.nz) return;
   const Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  // column of each x this lane handles in the x transform (loaded early)
+  // column of each x this lane handles in the x transform (loaded early);
+  // for R2C the upper half of a row is the conjugate of x' = N - x
   int uu[LX::E];
 #pragma unroll
-  for (int e = 0; e < LX::E; ++e) uu[e] = a.xCol[lane % LX::P + LX::P * e];
+  for (int e = 0; e < LX::E; ++e) {
+    const int x = lane % LX::P + LX::P * e;
+    uu[e] = a.xCol[(RC && 2 * x > N) ? N - x : x];
+  }
+  // R2C: hermitian fill of the plane's x = 0 column before its y transform
+  // (reference symmetry_host.hpp:47-58, same rule as the (0,0) stick), into
+  // LDS after the tile; the column's gathers read it from there.
+  const int tileA = S::ROWS_A * LU, tileB = S::ROWS_ALLOC * LB;
+  Cx<T>* fill = A + (tileA > tileB ? tileA : tileB);
+  const bool fillCol = RC && a.colX[0] == 0;
+  if (fillCol) {
+    for (int y = threadIdx.x; y < N; y += S::NT) {
+      const int yp = y == 0 ? 0 : N - y;
+      const int sa = a.stickAt[y], sb = a.stickAt[yp];
+      const Cx<T> xs = sa >= 0 ? zrow[sa] : Cx<T>{T(0), T(0)};
+      const Cx<T> xp = sb >= 0 ? zrow[sb] : Cx<T>{T(0), T(0)};
+      Cx<T> o = xs;
+      if (y != 0) {
+        if (yp == y) {
+          o = cnonzero(xs) ? cconj(xs) : xs;
+        } else if (y < yp) {
+          o = cnonzero(xs) ? xs : (cnonzero(xp) ? cconj(xp) : xs);
+        } else {
+          o = cnonzero(xp) ? cconj(xp) : xs;
+        }
+      }
+      fill[y] = o;
+    }
+    __syncthreads();
+  }
 
   // y: for each column u, decimation in frequency to the rows y = r + R m
   // (t[j] = W^(r j) sum_l X[j + M l] W_R^(r l)), then the M-point FFT. The
@@ -322,6 +373,24 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
 #pragma unroll
       for (int e = 0; e < LM::E; ++e) tcol[(q + LM::P * e) * LU] = v[e];
     }
+    if (RC && fillCol && wave == 0) {
+      // Redo column 0 (x = 0, computed above from the unfilled sticks) from
+      // the filled copy. Wave 0 owns column 0; all its lane groups compute
+      // the same line, so their LDS writes agree.
+      Cx<T> v[LM::E];
+#pragma unroll
+      for (int e = 0; e < LM::E; ++e) {
+        const int j = q + LM::P * e;
+        Cx<T> acc = fill[j];
+#pragma unroll
+        for (int l = 1; l < R; ++l) acc = cadd(acc, cmul(fill[j + M * l], wl[l]));
+        v[e] = cmul(acc, twiddle<true>(a.twY, r * j));
+      }
+      wave_sync();
+      wave_fft<T, M, true, R>(v, A, LU, q, a.twY);
+#pragma unroll
+      for (int e = 0; e < LM::E; ++e) A[(q + LM::P * e) * LU] = v[e];
+    }
   }
   __syncthreads();
 
@@ -332,23 +401,36 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
   Cx<T> v[LX::E];
 #pragma unroll
   for (int e = 0; e < LX::E; ++e) {
-    const Cx<T> x = A[mc * LU + (uu[e] >= 0 ? uu[e] : 0)];
-    v[e] = (m < M && uu[e] >= 0) ? x : Cx<T>{T(0), T(0)};
+    Cx<T> x = A[mc * LU + (uu[e] >= 0 ? uu[e] : 0)];
+    x = (m < M && uu[e] >= 0) ? x : Cx<T>{T(0), T(0)};
+    if (RC) {
+      // C2R: the imaginary parts of the DC and Nyquist inputs do not
+      // contribute to a real signal; the upper half is conjugated
+      const int xx = q + LX::P * e;
+      if (xx == 0 || 2 * xx == N) x.im = T(0);
+      if (2 * xx > N) x = cconj(x);
+    }
+    v[e] = x;
   }
   __syncthreads();  // A is dead from here on; its space holds the row lines
   if (wave * LX::PER_WAVE < M) {
     Cx<T>* line = A + m * LB;
     wave_fft<T, N, true, 1>(v, line, 1, q, a.twX);
     if (m < M) {
-      Cx<T>* out = reinterpret_cast<Cx<T>*>(a.out) +
-                   (static_cast<size_t>(z) * N + r + R * m) * N;
+      const size_t row = (static_cast<size_t>(z) * N + r + R * m) * N;
+      if (RC) {
 #pragma unroll
-      for (int e = 0; e < LX::E; ++e) out[q + LX::P * e] = v[e];
+        for (int e = 0; e < LX::E; ++e) a.out[row + q + LX::P * e] = v[e].re;
+      } else {
+        Cx<T>* out = reinterpret_cast<Cx<T>*>(a.out) + row;
+#pragma unroll
+        for (int e = 0; e < LX::E; ++e) out[q + LX::P * e] = v[e];
+      }
     }
   }
 }
 
-template <typename T, int N>
+template <typename T, int N, bool RC>
 __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
   using S = XYShape<T, N>;
   using LM = Line<S::M>;
@@ -377,15 +459,24 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
       const int y = y0 + lane / LM::P;
       const int yc = y < N ? y : N - 1;
       const Cx<T>* row = in + static_cast<size_t>(yc) * N;
+      const T* rrow = a.in + (static_cast<size_t>(z) * N + yc) * N;
       Cx<T> v[LM::E];
       constexpr int EC = LM::E % 4 == 0 ? 4 : LM::E;
 #pragma unroll
       for (int e0 = 0; e0 < LM::E; e0 += EC) {
         Cx<T> xv[EC][R];
+        if (RC) {
 #pragma unroll
-        for (int e = 0; e < EC; ++e) {
+          for (int e = 0; e < EC; ++e) {
 #pragma unroll
-          for (int l = 0; l < R; ++l) xv[e][l] = row[q + LM::P * (e0 + e) + MX * l];
+            for (int l = 0; l < R; ++l) xv[e][l] = Cx<T>{rrow[q + LM::P * (e0 + e) + MX * l], T(0)};
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < EC; ++e) {
+#pragma unroll
+            for (int l = 0; l < R; ++l) xv[e][l] = row[q + LM::P * (e0 + e) + MX * l];
+          }
         }
 #pragma unroll
         for (int e = 0; e < EC; ++e) {
@@ -436,27 +527,27 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
 // launch maps each workgroup to (transform, block) through `map`; a batched
 // xy launch gives every transform the same number of workgroups.
 
-template <typename T, int NZ>
+template <typename T, int NZ, bool RC>
 __global__ void __launch_bounds__((ZShape<T, NZ>::NT)) k_z_backward_fast(FastArgs<T> a) {
-  z_backward_body<T, NZ>(a, blockIdx.x);
+  z_backward_body<T, NZ, RC>(a, blockIdx.x);
 }
 template <typename T, int NZ>
 __global__ void __launch_bounds__((ZShape<T, NZ>::NT)) k_z_forward_fast(FastArgs<T> a) {
   z_forward_body<T, NZ>(a, blockIdx.x);
 }
-template <typename T, int N>
+template <typename T, int N, bool RC>
 __global__ void __launch_bounds__((XYShape<T, N>::NT)) k_xy_backward_fast(FastArgs<T> a) {
-  xy_backward_body<T, N>(a, blockIdx.x);
+  xy_backward_body<T, N, RC>(a, blockIdx.x);
 }
-template <typename T, int N>
+template <typename T, int N, bool RC>
 __global__ void __launch_bounds__((XYShape<T, N>::NT)) k_xy_forward_fast(FastArgs<T> a) {
-  xy_forward_body<T, N>(a, blockIdx.x);
+  xy_forward_body<T, N, RC>(a, blockIdx.x);
 }
-template <typename T, int NZ>
+template <typename T, int NZ, bool RC>
 __global__ void __launch_bounds__((ZShape<T, NZ>::NT))
     k_z_backward_batch(const FastArgs<T>* __restrict__ batch, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
-  z_backward_body<T, NZ>(batch[m.x], m.y);
+  z_backward_body<T, NZ, RC>(batch[m.x], m.y);
 }
 template <typename T, int NZ>
 __global__ void __launch_bounds__((ZShape<T, NZ>::NT))
@@ -464,17 +555,17 @@ __global__ void __launch_bounds__((ZShape<T, NZ>::NT))
   const int2 m = map[blockIdx.x];
   z_forward_body<T, NZ>(batch[m.x], m.y);
 }
-template <typename T, int N>
+template <typename T, int N, bool RC>
 __global__ void __launch_bounds__((XYShape<T, N>::NT))
     k_xy_backward_batch(const FastArgs<T>* __restrict__ batch, int perTransform) {
   const int t = blockIdx.x / perTransform;
-  xy_backward_body<T, N>(batch[t], blockIdx.x - t * perTransform);
+  xy_backward_body<T, N, RC>(batch[t], blockIdx.x - t * perTransform);
 }
-template <typename T, int N>
+template <typename T, int N, bool RC>
 __global__ void __launch_bounds__((XYShape<T, N>::NT))
     k_xy_forward_batch(const FastArgs<T>* __restrict__ batch, int perTransform) {
   const int t = blockIdx.x / perTransform;
-  xy_forward_body<T, N>(batch[t], blockIdx.x - t * perTransform);
+  xy_forward_body<T, N, RC>(batch[t], blockIdx.x - t * perTransform);
 }
 
 // Raise the dynamic LDS limit of a kernel once per process (the attribute
@@ -497,9 +588,9 @@ void prepare_lds(Kernel kernel, size_t lds) {
 template <typename T, int NZ>
 void z_backward_n(const FastArgs<T>& a, hipStream_t stream) {
   using Z = ZShape<T, NZ>;
-  auto kernel = k_z_backward_fast<T, NZ>;
+  auto kernel = a.r2c ? k_z_backward_fast<T, NZ, true> : k_z_backward_fast<T, NZ, false>;
   prepare_lds(kernel, Z::lds());
-  prof::KernelSpan span(stream, "k_z_backward_fast");
+  prof::KernelSpan span(stream, a.r2c ? "k_z_backward_r2c" : "k_z_backward_fast");
   hipLaunchKernelGGL(kernel, dim3((a.nst + Z::S - 1) / Z::S), dim3(Z::NT), Z::lds(), stream, a);
   check_launch();
 }
@@ -522,11 +613,11 @@ int plane_grid(int nz) {
 template <typename T, int N>
 void xy_backward_n(FastArgs<T> a, hipStream_t stream) {
   using S = XYShape<T, N>;
-  const size_t lds = S::lds_backward(a.ncol);
+  const size_t lds = S::lds_backward(a.ncol, a.r2c != 0);
   a.pitchA = S::pitch_a(a.ncol);
-  auto kernel = k_xy_backward_fast<T, N>;
+  auto kernel = a.r2c ? k_xy_backward_fast<T, N, true> : k_xy_backward_fast<T, N, false>;
   prepare_lds(kernel, lds);
-  prof::KernelSpan span(stream, "k_xy_backward_fast");
+  prof::KernelSpan span(stream, a.r2c ? "k_xy_backward_r2c" : "k_xy_backward_fast");
   hipLaunchKernelGGL(kernel, dim3(plane_grid<N>(a.nz)), dim3(S::NT), lds, stream, a);
   check_launch();
 }
@@ -535,16 +626,17 @@ template <typename T, int N>
 void xy_forward_n(const FastArgs<T>& a, hipStream_t stream) {
   using S = XYShape<T, N>;
   const size_t lds = S::lds_forward();
-  auto kernel = k_xy_forward_fast<T, N>;
+  auto kernel = a.r2c ? k_xy_forward_fast<T, N, true> : k_xy_forward_fast<T, N, false>;
   prepare_lds(kernel, lds);
-  prof::KernelSpan span(stream, "k_xy_forward_fast");
+  prof::KernelSpan span(stream, a.r2c ? "k_xy_forward_r2c" : "k_xy_forward_fast");
   hipLaunchKernelGGL(kernel, dim3(plane_grid<N>(a.nz)), dim3(S::NT), lds, stream, a);
   check_launch();
 }
 
 template <typename T, int N>
-bool fits(int ncol) {
-  return XYShape<T, N>::lds_backward(ncol) <= kLdsMax && XYShape<T, N>::lds_forward() <= kLdsMax;
+bool fits(int ncol, bool rc) {
+  return XYShape<T, N>::lds_backward(ncol, rc) <= kLdsMax &&
+         XYShape<T, N>::lds_forward() <= kLdsMax;
 }
 
 bool fast_length(int n) { return n == 32 || n == 64 || n == 96 || n == 128 || n == 192; }
@@ -562,32 +654,33 @@ bool fast_length(int n) { return n == 32 || n == 64 || n == 96 || n == 128 || n 
 }  // namespace
 
 template <typename T, int N>
-void xy_backward_batch_n(const FastArgs<T>* batch, int ntrans, int nz, int maxNcol,
+void xy_backward_batch_n(bool rc, const FastArgs<T>* batch, int ntrans, int nz, int maxNcol,
                          hipStream_t stream) {
   using S = XYShape<T, N>;
-  const size_t lds = S::lds_backward(maxNcol);
-  auto kernel = k_xy_backward_batch<T, N>;
+  const size_t lds = S::lds_backward(maxNcol, rc);
+  auto kernel = rc ? k_xy_backward_batch<T, N, true> : k_xy_backward_batch<T, N, false>;
   prepare_lds(kernel, lds);
-  prof::KernelSpan span(stream, "k_xy_backward_batch");
+  prof::KernelSpan span(stream, rc ? "k_xy_backward_batch_r2c" : "k_xy_backward_batch");
   const int per = plane_grid<N>(nz);
   hipLaunchKernelGGL(kernel, dim3(per * ntrans), dim3(S::NT), lds, stream, batch, per);
   check_launch();
 }
 
 template <typename T, int N>
-void xy_forward_batch_n(const FastArgs<T>* batch, int ntrans, int nz, hipStream_t stream) {
+void xy_forward_batch_n(bool rc, const FastArgs<T>* batch, int ntrans, int nz,
+                        hipStream_t stream) {
   using S = XYShape<T, N>;
   const size_t lds = S::lds_forward();
-  auto kernel = k_xy_forward_batch<T, N>;
+  auto kernel = rc ? k_xy_forward_batch<T, N, true> : k_xy_forward_batch<T, N, false>;
   prepare_lds(kernel, lds);
-  prof::KernelSpan span(stream, "k_xy_forward_batch");
+  prof::KernelSpan span(stream, rc ? "k_xy_forward_batch_r2c" : "k_xy_forward_batch");
   const int per = plane_grid<N>(nz);
   hipLaunchKernelGGL(kernel, dim3(per * ntrans), dim3(S::NT), lds, stream, batch, per);
   check_launch();
 }
 
 template <typename T, int NZ>
-void z_batch_n(bool forward, const FastArgs<T>* batch, const int2* map, int blocks,
+void z_batch_n(bool forward, bool rc, const FastArgs<T>* batch, const int2* map, int blocks,
                hipStream_t stream) {
   using Z = ZShape<T, NZ>;
   if (forward) {
@@ -596,9 +689,9 @@ void z_batch_n(bool forward, const FastArgs<T>* batch, const int2* map, int bloc
     prof::KernelSpan span(stream, "k_z_forward_batch");
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(Z::NT), Z::lds(), stream, batch, map);
   } else {
-    auto kernel = k_z_backward_batch<T, NZ>;
+    auto kernel = rc ? k_z_backward_batch<T, NZ, true> : k_z_backward_batch<T, NZ, false>;
     prepare_lds(kernel, Z::lds());
-    prof::KernelSpan span(stream, "k_z_backward_batch");
+    prof::KernelSpan span(stream, rc ? "k_z_backward_batch_r2c" : "k_z_backward_batch");
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(Z::NT), Z::lds(), stream, batch, map);
   }
   check_launch();
@@ -615,10 +708,10 @@ void fast_z_geometry(int nz, int* sticksPerBlock, int* lineLength, int* padShift
 }
 
 template <typename T>
-bool fast_path_supported(int nx, int ny, int nz, int ncol) {
+bool fast_path_supported(int nx, int ny, int nz, int ncol, bool r2c) {
   if (nx != ny || !fast_length(nx) || !fast_length(nz) || ncol < 1) return false;
   bool ok = false;
-#define SPFFT_FITS(N) ok = fits<T, N>(ncol)
+#define SPFFT_FITS(N) ok = fits<T, N>(ncol, r2c)
   SPFFT_FAST_SWITCH(nx, SPFFT_FITS)
 #undef SPFFT_FITS
   return ok;
@@ -664,22 +757,22 @@ int fast_xy_pitch_a(int n, int ncol) {
 }
 
 template <typename T>
-void launch_fast_z_batch(bool forward, int nz, const FastArgs<T>* batch, const int2* map,
-                         int blocks, hipStream_t stream) {
-#define SPFFT_CALL(N) z_batch_n<T, N>(forward, batch, map, blocks, stream)
+void launch_fast_z_batch(bool forward, bool r2c, int nz, const FastArgs<T>* batch,
+                         const int2* map, int blocks, hipStream_t stream) {
+#define SPFFT_CALL(N) z_batch_n<T, N>(forward, r2c, batch, map, blocks, stream)
   SPFFT_FAST_SWITCH(nz, SPFFT_CALL)
 #undef SPFFT_CALL
 }
 
 template <typename T>
-void launch_fast_xy_batch(bool forward, int n, const FastArgs<T>* batch, int ntrans, int nz,
-                          int maxNcol, hipStream_t stream) {
+void launch_fast_xy_batch(bool forward, bool r2c, int n, const FastArgs<T>* batch, int ntrans,
+                          int nz, int maxNcol, hipStream_t stream) {
   if (forward) {
-#define SPFFT_CALL(N) xy_forward_batch_n<T, N>(batch, ntrans, nz, stream)
+#define SPFFT_CALL(N) xy_forward_batch_n<T, N>(r2c, batch, ntrans, nz, stream)
     SPFFT_FAST_SWITCH(n, SPFFT_CALL)
 #undef SPFFT_CALL
   } else {
-#define SPFFT_CALL(N) xy_backward_batch_n<T, N>(batch, ntrans, nz, maxNcol, stream)
+#define SPFFT_CALL(N) xy_backward_batch_n<T, N>(r2c, batch, ntrans, nz, maxNcol, stream)
     SPFFT_FAST_SWITCH(n, SPFFT_CALL)
 #undef SPFFT_CALL
   }
@@ -687,11 +780,11 @@ void launch_fast_xy_batch(bool forward, int n, const FastArgs<T>* batch, int ntr
 
 #define SPFFT_INSTANTIATE_FAST(T)                                                        \
   template int fast_xy_pitch_a<T>(int, int);                                             \
-  template void launch_fast_z_batch<T>(bool, int, const FastArgs<T>*, const int2*, int,  \
-                                       hipStream_t);                                     \
-  template void launch_fast_xy_batch<T>(bool, int, const FastArgs<T>*, int, int, int,    \
-                                        hipStream_t);                                    \
-  template bool fast_path_supported<T>(int, int, int, int);                              \
+  template void launch_fast_z_batch<T>(bool, bool, int, const FastArgs<T>*, const int2*, \
+                                       int, hipStream_t);                                \
+  template void launch_fast_xy_batch<T>(bool, bool, int, const FastArgs<T>*, int, int,   \
+                                        int, hipStream_t);                               \
+  template bool fast_path_supported<T>(int, int, int, int, bool);                        \
   template void fast_z_geometry<T>(int, int*, int*, int*);                               \
   template void launch_fast_z_backward<T>(int, const FastArgs<T>&, hipStream_t);         \
   template void launch_fast_z_forward<T>(int, const FastArgs<T>&, hipStream_t);          \

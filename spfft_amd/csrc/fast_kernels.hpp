@@ -1,5 +1,5 @@
-// Fixed-length fast path of the C2C sparse 3D FFT (lengths with compile-time
-// radix schedules). Two kernels per direction:
+// Fixed-length fast path of the sparse 3D FFT (lengths with compile-time
+// radix schedules), C2C and R2C. Two kernels per direction:
 //
 //   z kernel  : sparse values -> LDS sticks -> z FFT -> zbuf[z][stick]
 //               (forward: zbuf -> z FFT -> gather/scale into user order)
@@ -14,6 +14,10 @@
 //       class's columns; results are gathered at the stick positions.
 //   Each class reads the whole plane input (from L2 after the first class);
 //   the classes of a plane are mapped to the same XCD.
+//   R2C (a compile-time variant): the backward z kernel fills the (0,0)
+//   stick hermitian, the backward xy kernel fills the x = 0 column of each
+//   plane and completes every row from its half spectrum before a complex x
+//   FFT whose real part is stored; the forward xy kernel reads real rows.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,6 +33,8 @@ struct FastArgs {
   int nz = 0, nst = 0, ncol = 0;
   int pitch = 0;   // row pitch of zbuf in elements (nst rounded up to 128 bytes)
   int pitchA = 0;  // row pitch of the backward xy kernel's column tile (set at launch)
+  int r2c = 0;            // real space domain, hermitian half in x
+  int zeroZeroStick = 0;  // 0 when the (x=0, y=0) stick exists
   Cx<T>* zbuf = nullptr;
   const T* in = nullptr;
   T* out = nullptr;
@@ -70,7 +76,7 @@ inline int fast_pitch(int nst) {
 
 // True when (nx, ny, nz) have fast kernels and their LDS fits for ncol columns.
 template <typename T>
-bool fast_path_supported(int nx, int ny, int nz, int ncol);
+bool fast_path_supported(int nx, int ny, int nz, int ncol, bool r2c);
 
 // Row pitch of the backward xy kernel's column tile for ncol columns.
 template <typename T>
@@ -78,12 +84,13 @@ int fast_xy_pitch_a(int n, int ncol);
 
 // Batched launches over transforms of one shape. `batch` is a device table of
 // arguments; `map` gives (transform, block) for every z workgroup.
+// All transforms of a batch have the same type (r2c).
 template <typename T>
-void launch_fast_z_batch(bool forward, int nz, const FastArgs<T>* batch, const int2* map,
-                         int blocks, hipStream_t stream);
+void launch_fast_z_batch(bool forward, bool r2c, int nz, const FastArgs<T>* batch,
+                         const int2* map, int blocks, hipStream_t stream);
 template <typename T>
-void launch_fast_xy_batch(bool forward, int n, const FastArgs<T>* batch, int ntrans, int nz,
-                          int maxNcol, hipStream_t stream);
+void launch_fast_xy_batch(bool forward, bool r2c, int n, const FastArgs<T>* batch, int ntrans,
+                          int nz, int maxNcol, hipStream_t stream);
 
 template <typename T>
 void launch_fast_z_backward(int nz, const FastArgs<T>& a, hipStream_t stream);

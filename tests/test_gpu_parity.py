@@ -13,7 +13,7 @@ import pytest
 
 import spfft_amd as sp
 from oracle import oracle as orc
-from tests.workloads import complex_values, random_index_set, sphere_triplets, uniform
+from tests.workloads import complex_values, random_index_set, sphere_triplets, uniform, uniform01
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -269,27 +269,29 @@ def test_examples_run():
 FAST_LENGTHS = [32, 64, 96, 128, 192]
 
 
+@pytest.mark.parametrize("r2c", [False, True], ids=["c2c", "r2c"])
 @pytest.mark.parametrize("precision", ["double", "float"])
 @pytest.mark.parametrize("n", FAST_LENGTHS)
 @pytest.mark.parametrize("fill", [(0.7, 0.7, False), (0.15, 0.5, True), (1.0, 1.0, True)],
                          ids=["dense70", "sparse_centred", "full"])
-def test_fixed_length_path_random_sets(n, fill, precision):
-    """Random index sets on the fixed-length C2C kernels (every column pattern,
-    including fully dense planes)."""
+def test_fixed_length_path_random_sets(n, fill, precision, r2c):
+    """Random index sets on the fixed-length kernels (every column pattern,
+    including fully dense planes). For R2C the values are arbitrary on the
+    half space, so the stick and plane symmetry fills decide the result."""
     sf, ff, centered = fill
-    if n == 192 and sf == 1.0:
+    if n == 192 and sf == 1.0 and not r2c:
         pytest.skip("dense 192^3 column set exceeds the fixed-length LDS budget (generic path)")
     dims = (n, n, n)
-    t = random_index_set(70 + n, dims, False, stick_fraction=sf, fill_fraction=ff,
+    t = random_index_set(70 + n, dims, r2c, stick_fraction=sf, fill_fraction=ff,
                          centered=centered)
     t = t[np.argsort(uniform(71, len(t)))]  # arbitrary value order
     v = complex_values(72, len(t))
-    o = orc.OraclePlan(False, dims, t)
-    tr = build(dims, t, False, precision)
+    o = orc.OraclePlan(r2c, dims, t)
+    tr = build(dims, t, r2c, precision)
     tol = FP64_TOL if precision == "double" else FP32_TOL
     want = o.backward(v)
-    assert orc.rel_l2(gpu_backward(tr, v, dims, False, precision), want) <= tol
-    f = gpu_forward(tr, want, False, sp.SPFFT_FULL_SCALING, precision)
+    assert orc.rel_l2(gpu_backward(tr, v, dims, r2c, precision), want) <= tol
+    f = gpu_forward(tr, want, r2c, sp.SPFFT_FULL_SCALING, precision)
     assert orc.rel_l2(f, o.forward(want, True)) <= tol
 
 
@@ -301,7 +303,7 @@ def test_batched_multi_transform_uneven_sets(n, precision):
     oracle and the single-transform call."""
     dims = (n, n, n)
     rng_k = uniform(80 + n, 3 * 6).reshape(6, 3) * 0.5
-    radii = 0.22 * n + 0.08 * n * uniform01_like(81 + n, 6)
+    radii = 0.22 * n + 0.08 * n * uniform01(81 + n, 6)
     trs, sets, vals = [], [], []
     for k in range(6):
         t = sphere_triplets(float(radii[k]), kshift=tuple(rng_k[k]))
@@ -327,6 +329,50 @@ def test_batched_multi_transform_uneven_sets(n, precision):
         assert orc.rel_l2(f, o.forward(want, True)) <= tol, k
 
 
-def uniform01_like(seed, n):
-    from tests.workloads import uniform01
-    return uniform01(seed, n)
+@pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("n", [32, 96, 128])
+def test_fixed_length_r2c_symmetry_fill_branches(n, precision):
+    """R2C sets holding both halves of the x = 0 plane and the whole (0,0)
+    stick, with a third of the values exactly zero, so every branch of the
+    symmetry fills is taken (lower wins, upper fills a zero lower, both zero,
+    self-paired Nyquist entries)."""
+    dims = (n, n, n)
+    nxf = n // 2 + 1
+    x, y, z = np.meshgrid(np.arange(nxf), np.arange(n), np.arange(n), indexing="ij")
+    keep = (uniform01(200 + n, nxf * n * n).reshape(nxf, n, n) < 0.4) | ((x == 0) & (y < 3))
+    t = np.ascontiguousarray(np.stack([x[keep], y[keep], z[keep]], axis=1).astype(np.int32))
+    v = complex_values(201 + n, len(t))
+    v[uniform01(202 + n, len(t)) < 0.33] = 0.0
+    o = orc.OraclePlan(True, dims, t)
+    tr = build(dims, t, True, precision)
+    tol = FP64_TOL if precision == "double" else FP32_TOL
+    assert orc.rel_l2(gpu_backward(tr, v, dims, True, precision), o.backward(v)) <= tol
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_batched_multi_transform_r2c(precision):
+    """A batch of R2C half-sphere transforms with shifted centres on the fused
+    launch chain."""
+    n = 96
+    dims = (n, n, n)
+    trs, sets, vals = [], [], []
+    for k in range(4):
+        t = sphere_triplets(20.0 + 3 * k, half=True)
+        trs.append(build(dims, t, True, precision))
+        sets.append(t)
+        vals.append(complex_values(300 + k, len(t)))
+    dt = torch.float64 if precision == "double" else torch.float32
+    rd = rdt(precision)
+    ins = [dev(v.view(np.float64).astype(rd)) for v in vals]
+    spaces = [torch.empty(n ** 3, dtype=dt, device="cuda") for _ in trs]
+    outs = [torch.empty(2 * len(t), dtype=dt, device="cuda") for t in sets]
+    sp.multi_transform_backward(trs, ins, spaces)
+    sp.multi_transform_forward(trs, spaces, outs, [sp.SPFFT_FULL_SCALING] * len(trs))
+    tol = FP64_TOL if precision == "double" else FP32_TOL
+    for k, t in enumerate(sets):
+        o = orc.OraclePlan(True, dims, t)
+        want = o.backward(vals[k])
+        got = spaces[k].cpu().numpy().astype(np.float64).reshape(n, n, n)
+        assert orc.rel_l2(got, want) <= tol, k
+        f = outs[k].cpu().numpy().astype(np.float64).view(np.complex128)
+        assert orc.rel_l2(f, o.forward(want, True)) <= tol, k
