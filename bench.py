@@ -86,49 +86,175 @@ def cpu_baseline(t, dims, values, seconds):
                       f"({el:.1f} s), C/OpenMP restatement of the reference host path"}
 
 
-def batched_bench(args, sp, torch, np, t, dims, ns, ng, plain_value):
-    """Batched form: --batch bands of the same 128^3 r=40 set through one
-    multi_transform_backward/forward call each, FP64 and FP32; band-pairs/s
-    and the FP32 / plain-FP64 ratio. Device-resident, values as above."""
-    from tests.workloads import complex_values
-    out = {"bands": args.batch}
-    for prec in ("double", "float"):
-        real = np.float64 if prec == "double" else np.float32
-        cls = sp.Transform if prec == "double" else sp.TransformFloat
-        trs, ins, outs = [], [], []
-        for k in range(args.batch):
-            tr = cls(-1, sp.SPFFT_PU_GPU, sp.SPFFT_TRANS_C2C, *dims, ns, sp.SPFFT_INDEX_TRIPLETS, t)
-            tr.set_execution_mode(sp.SPFFT_EXEC_ASYNCHRONOUS)
-            v = complex_values(0x5EED0100 + k, ns)
-            ins.append(torch.from_numpy(v.view(np.float64).astype(real)).cuda())
-            outs.append(torch.empty(2 * ns, dtype=ins[-1].dtype, device="cuda"))
-            trs.append(tr)
-        locs = [sp.SPFFT_PU_GPU] * args.batch
-        scal = [sp.SPFFT_FULL_SCALING] * args.batch
+def _bench_transforms(sp, torch, np, trs, ins, outs, steps, r2c=False):
+    """Seconds per multi_transform backward+forward call over all transforms."""
+    locs = [sp.SPFFT_PU_GPU] * len(trs)
+    scal = [sp.SPFFT_FULL_SCALING] * len(trs)
 
-        def step():
-            sp.multi_transform_backward(trs, ins, locs)
-            sp.multi_transform_forward(trs, locs, outs, scal)
+    def step():
+        sp.multi_transform_backward(trs, ins, locs)
+        sp.multi_transform_forward(trs, locs, outs, scal)
 
-        for _ in range(3):
-            step()
-        torch.cuda.synchronize()
-        steps = max(5, args.steps // args.batch)
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            step()
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        rate = steps * args.batch / el
-        rb = np.dtype(real).itemsize
-        pair_bytes = 2 * (ns * (2 * rb + 4) + ng * 2 * rb)
-        out[prec] = {"band_pairs_per_s": rate, "ms_per_call_pair": el / steps * 1e3,
-                     "frac_hbm": rate * pair_bytes / 1e9 / HBM_PEAK_GBS}
-        log(f"batched {prec}: {rate:.1f} band-pairs/s ({args.batch} bands per call)")
-        del trs, ins, outs
-        torch.cuda.empty_cache()
-    out["fp32_vs_plain_fp64"] = out["float"]["band_pairs_per_s"] / plain_value
-    out["fp64_vs_plain_fp64"] = out["double"]["band_pairs_per_s"] / plain_value
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def batched_bench(args, sp, torch, np):
+    """Batched forms, each one multi_transform_backward + _forward call over
+    all bands (one fused launch chain per direction), device-resident:
+      - 128^3, sphere r=40, 64 bands, FP64 and FP32 (the FP32 / plain-FP64
+        ratio is taken on this one);
+      - 96^3, sphere r=30, 64 bands, FP64 and FP32, values N(0,1) exp(-|G|^2/900);
+      - 96^3, 16 k-points x 32 bands, each k-point its own sphere |G+k|^2 <= r^2
+        with k ~ U[-1/2,1/2)^3 and r ~ U[22,30] (uneven sets), FP32.
+    Band-pairs/s and the fraction of the HBM roofline (compulsory bytes:
+    values + map + dense grid per direction)."""
+    from tests.workloads import complex_values, normal, sphere_triplets, uniform, uniform01
+    out = {}
+    cases = [("128^3 r=40 x64", 128, [40.0], [(0.0, 0.0, 0.0)], 64, ("double", "float"), "uniform"),
+             ("96^3 r=30 x64", 96, [30.0], [(0.0, 0.0, 0.0)], 64, ("double", "float"), "gauss")]
+    ks = uniform(0x5EED0004, 48).reshape(16, 3) * 0.5
+    rs = 22.0 + 8.0 * uniform01(0x5EED0005, 16)
+    cases.append(("96^3 16 k-points x32", 96, list(rs), [tuple(k) for k in ks], 32, ("float",),
+                  "gauss"))
+    for name, n, radii, kpts, bands, precs, dist in cases:
+        if args.batch_cases and not any(c in name for c in args.batch_cases):
+            continue
+        dims = (n, n, n)
+        sets = [sphere_triplets(r, kshift=k) for r, k in zip(radii, kpts)]
+        rec = {"grid": n, "bands_per_set": bands, "sets": len(sets),
+               "values_per_set": [len(t) for t in sets]}
+        for prec in precs:
+            real = np.float64 if prec == "double" else np.float32
+            cls = sp.Transform if prec == "double" else sp.TransformFloat
+            trs, ins, outs = [], [], []
+            for si, t in enumerate(sets):
+                g2 = np.sum(t.astype(np.float64) ** 2, axis=1)
+                for b in range(bands):
+                    tr = cls(-1, sp.SPFFT_PU_GPU, sp.SPFFT_TRANS_C2C, *dims, len(t),
+                             sp.SPFFT_INDEX_TRIPLETS, t)
+                    tr.set_execution_mode(sp.SPFFT_EXEC_ASYNCHRONOUS)
+                    seed = 0x5EED0100 + 1000 * si + b
+                    if dist == "uniform":
+                        v = complex_values(seed, len(t))
+                    else:
+                        w = np.exp(-g2 / 900.0)
+                        v = (normal(seed, len(t)) + 1j * normal(seed + 7919, len(t))) * w
+                    ins.append(torch.from_numpy(v.view(np.float64).astype(real)).cuda())
+                    outs.append(torch.empty(2 * len(t), dtype=ins[-1].dtype, device="cuda"))
+                    trs.append(tr)
+            steps = max(5, args.steps // len(trs))
+            sec = _bench_transforms(sp, torch, np, trs, ins, outs, steps)
+            rb = np.dtype(real).itemsize
+            nv = sum(len(t) for t in sets) * bands
+            pair_bytes = 2 * (nv * (2 * rb + 4) + len(trs) * n ** 3 * 2 * rb)
+            rate = len(trs) / sec
+            rec[prec] = {"band_pairs_per_s": rate, "ms_per_call_pair": sec * 1e3,
+                         "frac_hbm": pair_bytes / sec / 1e9 / HBM_PEAK_GBS}
+            log(f"batched {name} {prec}: {rate:.1f} band-pairs/s, "
+                f"{rec[prec]['frac_hbm'] * 100:.1f}% of HBM roofline")
+            del trs, ins, outs
+            torch.cuda.empty_cache()
+        out[name] = rec
+    return out
+
+
+def plain_r2c_bench(args, sp, torch, np):
+    """128^3 R2C FP64, half sphere r=40 (x > 0, or x = 0 and y > 0, or
+    x = y = 0 and z >= 0): single transforms, backward+forward pairs/s."""
+    from tests.workloads import complex_values, sphere_triplets
+    dims = (128, 128, 128)
+    t = sphere_triplets(40, half=True)
+    trs, ins, outs = [], [], []
+    for k in range(args.rotate):
+        tr = sp.Transform(-1, sp.SPFFT_PU_GPU, sp.SPFFT_TRANS_R2C, *dims, len(t),
+                          sp.SPFFT_INDEX_TRIPLETS, t)
+        tr.set_execution_mode(sp.SPFFT_EXEC_ASYNCHRONOUS)
+        ins.append(torch.from_numpy(complex_values(0x5EED0003 + k, len(t)).view(np.float64)).cuda())
+        outs.append(torch.empty(2 * len(t), dtype=torch.float64, device="cuda"))
+        trs.append(tr)
+
+    def pair(i):
+        k = i % args.rotate
+        trs[k].backward(ins[k].data_ptr(), sp.SPFFT_PU_GPU)
+        trs[k].forward(sp.SPFFT_PU_GPU, outs[k].data_ptr(), sp.SPFFT_FULL_SCALING)
+
+    for i in range(args.warmup):
+        pair(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        pair(i)
+    torch.cuda.synchronize()
+    rate = args.steps / (time.perf_counter() - t0)
+    pair_bytes = 2 * (len(t) * 20 + 128 ** 3 * 8)
+    log(f"plain R2C 128^3 r=40: {rate:.1f} pairs/s")
+    return {"values": len(t), "pairs_per_s": rate,
+            "frac_hbm_wall": pair_bytes * rate / 1e9 / HBM_PEAK_GBS}
+
+
+def _counter_rows(folder):
+    """(kernel name, counter value) rows of a rocprofv3 --pmc csv output."""
+    import csv
+    import glob
+    rows = []
+    for f in sorted(glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True)):
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                name = row["Kernel_Name"]
+                for pre in ("void spfft::(anonymous namespace)::", "spfft::(anonymous namespace)::"):
+                    name = name.replace(pre, "")
+                rows.append((name, float(row["Counter_Value"])))
+    return rows
+
+
+def traffic_summary(fetch_dir, write_dir, copy_mib, precision, command):
+    """Measured HBM bytes per launch from two rocprofv3 --pmc runs of this
+    script (FETCH_SIZE, then WRITE_SIZE, both in KiB per dispatch), each
+    scaled by what the counter reads for the device-to-device copy of
+    --copy-mib MiB that the profiled runs include (the runtime's plain copy
+    kernel: it reads and writes exactly that many bytes). Writes
+    profiles/traffic_<kernel>_<precision>.json for the library's kernels."""
+    import statistics
+    expect_kib = copy_mib * 1024.0
+    per = {}
+    scale = {}
+    for counter, folder in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = _counter_rows(folder)
+        copies = [v for n, v in rows if "copyBuffer" in n]
+        if not copies:
+            raise SystemExit(f"no copy kernel dispatches under {folder}")
+        big = [v for v in copies if v >= 0.5 * max(copies)]
+        scale[counter] = expect_kib / statistics.median(big)
+        for n, v in rows:
+            if not n.startswith("k_"):
+                continue
+            base = n.split("<")[0]
+            if ", true>" in n.split("(")[0]:  # R2C variants are profiled as <name>_r2c
+                base = base.replace("_fast", "") + "_r2c"
+            per.setdefault(base, {}).setdefault(counter, []).append(v)
+    out = {}
+    for k, d in per.items():
+        if "FETCH_SIZE" not in d or "WRITE_SIZE" not in d:
+            continue
+        f = statistics.mean(d["FETCH_SIZE"])
+        w = statistics.mean(d["WRITE_SIZE"])
+        out[k] = {
+            "fetch_kib_per_dispatch": f, "write_kib_per_dispatch": w,
+            "fetch_scale": scale["FETCH_SIZE"], "write_scale": scale["WRITE_SIZE"],
+            "bytes_per_launch": 1024.0 * (f * scale["FETCH_SIZE"] + w * scale["WRITE_SIZE"]),
+            "dispatches": len(d["FETCH_SIZE"]), "copy_mib": copy_mib, "command": command,
+        }
+        with open(os.path.join(ROOT, "profiles", f"traffic_{k}_{precision}.json"), "w") as fh:
+            json.dump(out[k], fh, indent=1)
+        log(f"{k:28s} fetch {f:10.1f} KiB x{scale['FETCH_SIZE']:.2f}  write {w:10.1f} KiB "
+            f"x{scale['WRITE_SIZE']:.2f}  -> {out[k]['bytes_per_launch'] / 1e6:8.2f} MB/launch")
     return out
 
 
@@ -141,9 +267,23 @@ def main():
     ap.add_argument("--precision", choices=["double", "float"], default="double")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batch", type=int, default=64,
-                    help="transforms per multi_transform call in the batched section (0: skip)")
+    ap.add_argument("--batch", type=int, default=1,
+                    help="run the batched section (0: skip)")
+    ap.add_argument("--batch-cases", nargs="*", default=[],
+                    help="substrings selecting batched cases (default: all)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the measurements, 10 device-to-device copies of this many MiB "
+                         "(the calibration dispatches for --pmc FETCH_SIZE / WRITE_SIZE runs)")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise two rocprofv3 --pmc outputs (FETCH_SIZE, WRITE_SIZE) of "
+                         "runs with --copy-mib into profiles/traffic_*.json and exit")
+    ap.add_argument("--traffic-command", default="",
+                    help="the profiling command recorded in the traffic summaries")
     args = ap.parse_args()
+    if args.traffic:
+        traffic_summary(args.traffic[0], args.traffic[1], args.copy_mib or 256, args.precision,
+                        args.traffic_command)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched "
               "path shards by bands/k-points with no communication and is not run on "
@@ -263,14 +403,27 @@ def main():
         },
         "kernels": kernels,
     }
+    result["plain_r2c"] = plain_r2c_bench(args, sp, torch, np)
     if args.batch > 0:
-        result["batched"] = batched_bench(args, sp, torch, np, t, dims, ns, ng, value)
+        b = batched_bench(args, sp, torch, np)
+        main_case = b.get("128^3 r=40 x64")
+        if main_case:
+            b["fp32_vs_plain_fp64"] = main_case["float"]["band_pairs_per_s"] / value
+            b["fp64_vs_plain_fp64"] = main_case["double"]["band_pairs_per_s"] / value
+        result["batched"] = b
     if not args.no_cpu_baseline:
         v0 = complex_values(0x5EED0002, ns)
         result["cpu_baseline"] = cpu_baseline(t, dims, v0, args.cpu_seconds)
         log(f"cpu baseline: {result['cpu_baseline']['value']:.2f} pairs/s on "
             f"{result['cpu_baseline']['cores']} threads")
     print(json.dumps(result), flush=True)
+    if args.copy_mib > 0:
+        n = args.copy_mib << 20
+        src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

@@ -1,11 +1,16 @@
 #include "execution.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <cstdint>
 #include <cstdlib>
+#include <list>
+#include <mutex>
 #include <utility>
 #include <vector>
 
 #include "hip_check.hpp"
+#include "profiling.hpp"
 #include "spfft/exceptions.hpp"
 
 namespace spfft {
@@ -37,6 +42,48 @@ bool disable_fast_path() {
   return v && v[0] == '1';
 }
 
+// SPFFT_AMD_GRAPHS=0 replays batched calls with direct launches instead of
+// captured HIP graphs.
+bool graphs_enabled() {
+  const char* v = std::getenv("SPFFT_AMD_GRAPHS");
+  return !(v && v[0] == '0');
+}
+
+std::atomic<std::uint64_t> next_serial{1};
+
+// A batched call's device argument table, block map and captured launch
+// chain, keyed by everything they depend on: direction, device, and per
+// transform its serial number (never reused, so a destroyed transform's plan
+// can never match again), its device input/output pointers and scaling.
+struct BatchPlan {
+  std::vector<std::uint64_t> key;
+  void* args = nullptr;
+  void* map = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+};
+
+// Most recently used first. Heap-allocated and never destroyed: plans hold
+// device memory that must not be released after the runtime shuts down.
+struct BatchCache {
+  std::mutex mutex;
+  std::list<BatchPlan> plans;
+};
+constexpr size_t kMaxBatchPlans = 8;
+
+BatchCache& batch_cache() {
+  static BatchCache* cache = new BatchCache();
+  return *cache;
+}
+
+void release_plan(BatchPlan& p) {
+  if (p.exec) (void)hipGraphExecDestroy(p.exec);
+  if (p.graph) (void)hipGraphDestroy(p.graph);
+  if (p.args) (void)hipFree(p.args);
+  if (p.map) (void)hipFree(p.map);
+  p = BatchPlan{};
+}
+
 }  // namespace
 
 template <typename T>
@@ -47,6 +94,7 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
       scale_(static_cast<T>(1.0 / static_cast<double>(static_cast<unsigned long long>(map->dimX) *
                                                       static_cast<unsigned long long>(map->dimY) *
                                                       static_cast<unsigned long long>(map->dimZ)))) {
+  serial_ = next_serial.fetch_add(1);
   check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   check_hip(hipEventCreateWithFlags(&startEvent_, hipEventDisableTiming));
   check_hip(hipEventCreateWithFlags(&endEvent_, hipEventDisableTiming));
@@ -74,8 +122,11 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
           fast_path_supported<T>(map_->dimX, map_->dimY, map_->dimZ, numCols_,
                                  map_->type == SPFFT_TRANS_R2C);
   if (fast_) {
+    // Stick index at (column, y); absent positions, and the extra row
+    // `numCols_` used by the lanes past the last column, point at the zero
+    // element that every row of the [z][stick] buffer holds at index nst.
     const int ny = map_->dimY;
-    std::vector<int> at(static_cast<size_t>(numCols_) * ny, -1);
+    std::vector<int> at(static_cast<size_t>(numCols_ + 1) * ny, map_->num_sticks());
     for (int s = 0; s < map_->num_sticks(); ++s) {
       at[static_cast<size_t>(t.stickCol[s]) * ny + t.stickY[s]] = s;
     }
@@ -110,7 +161,8 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
       resYM[i] = (t.stickY[st] << 16) | (t.colX[t.stickCol[st]] / classes);
     }
     resYM_ = DeviceBuffer<int>::from_host(resYM);
-    std::vector<int> xCol(map_->dimX, -1);
+    // column of x, or the all-zero column numCols_ of the xy tile
+    std::vector<int> xCol(map_->dimX, numCols_);
     for (int u = 0; u < numCols_; ++u) xCol[t.colX[u]] = u;
     xCol_ = DeviceBuffer<int>::from_host(xCol);
     resSticks_ = DeviceBuffer<int>::from_host(resSticks);
@@ -340,47 +392,113 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
       return false;
     }
   }
-  int S = 0, LZ = 0, shift = 0;
-  fast_z_geometry<T>(m0.dimZ, &S, &LZ, &shift);
-  std::vector<FastArgs<T>> args(n);
-  std::vector<int2> zmap;
-  int maxNcol = 0;
+  std::vector<const T*> din(n);
+  std::vector<T*> dout(n);
   for (size_t t = 0; t < n; ++t) {
     auto i = translate_pointer(in[t]);
     auto o = translate_pointer(out[t]);
     if (!i.second || !o.second) return false;
-    FastArgs<T> f = ex[t]->fast_args();
-    f.in = i.second;
-    f.out = o.second;
-    f.useScale = forward && scaling[t] == SPFFT_FULL_SCALING ? 1 : 0;
-    f.pitchA = fast_xy_pitch_a<T>(m0.dimX, f.ncol);
-    maxNcol = std::max(maxNcol, f.ncol);
-    const int blocks = (f.nst + S - 1) / S;
-    for (int b = 0; b < blocks; ++b) zmap.push_back(make_int2(static_cast<int>(t), b));
-    args[t] = f;
+    din[t] = i.second;
+    dout[t] = o.second;
   }
   if (hipGetLastError() != hipSuccess) throw GPUPrecedingError();
-  // Argument tables live in stream-ordered allocations on the default stream.
-  FastArgs<T>* dArgs = nullptr;
-  int2* dMap = nullptr;
-  check_hip(hipMallocAsync(reinterpret_cast<void**>(&dArgs), n * sizeof(FastArgs<T>), nullptr));
-  check_hip(hipMallocAsync(reinterpret_cast<void**>(&dMap), zmap.size() * sizeof(int2), nullptr));
-  check_hip(hipMemcpyAsync(dArgs, args.data(), n * sizeof(FastArgs<T>), hipMemcpyHostToDevice,
-                           nullptr));
-  check_hip(hipMemcpyAsync(dMap, zmap.data(), zmap.size() * sizeof(int2), hipMemcpyHostToDevice,
-                           nullptr));
-  const int blocks = static_cast<int>(zmap.size());
+
+  std::vector<std::uint64_t> key;
+  key.reserve(3 + 4 * n);
+  key.push_back(forward ? 1 : 0);
+  key.push_back(static_cast<std::uint64_t>(ex[0]->grid_->device_id()));
+  key.push_back(n);
+  for (size_t t = 0; t < n; ++t) {
+    key.push_back(ex[t]->serial_);
+    key.push_back(reinterpret_cast<std::uintptr_t>(din[t]));
+    key.push_back(reinterpret_cast<std::uintptr_t>(dout[t]));
+    key.push_back(forward && scaling[t] == SPFFT_FULL_SCALING ? 1 : 0);
+  }
+
+  int S = 0, LZ = 0, shift = 0;
+  fast_z_geometry<T>(m0.dimZ, &S, &LZ, &shift);
+  int blocks = 0, maxNcol = 0;
+  for (size_t t = 0; t < n; ++t) {
+    blocks += (ex[t]->map_->num_sticks() + S - 1) / S;
+    maxNcol = std::max(maxNcol, ex[t]->numCols_);
+  }
   const bool r2c = m0.type == SPFFT_TRANS_R2C;
   const int nt = static_cast<int>(n);
-  if (forward) {
-    launch_fast_xy_batch<T>(true, r2c, m0.dimX, dArgs, nt, m0.dimZ, maxNcol, nullptr);
-    launch_fast_z_batch<T>(true, r2c, m0.dimZ, dArgs, dMap, blocks, nullptr);
-  } else {
-    launch_fast_z_batch<T>(false, r2c, m0.dimZ, dArgs, dMap, blocks, nullptr);
-    launch_fast_xy_batch<T>(false, r2c, m0.dimX, dArgs, nt, m0.dimZ, maxNcol, nullptr);
+  auto launch_chain = [&](const FastArgs<T>* dArgs, const int2* dMap, hipStream_t s) {
+    if (forward) {
+      launch_fast_xy_batch<T>(true, r2c, m0.dimX, dArgs, nt, m0.dimZ, maxNcol, s);
+      launch_fast_z_batch<T>(true, r2c, m0.dimZ, dArgs, dMap, blocks, s);
+    } else {
+      launch_fast_z_batch<T>(false, r2c, m0.dimZ, dArgs, dMap, blocks, s);
+      launch_fast_xy_batch<T>(false, r2c, m0.dimX, dArgs, nt, m0.dimZ, maxNcol, s);
+    }
+  };
+
+  BatchCache& cache = batch_cache();
+  std::lock_guard<std::mutex> lock(cache.mutex);
+  auto hit = std::find_if(cache.plans.begin(), cache.plans.end(),
+                          [&](const BatchPlan& p) { return p.key == key; });
+  if (hit == cache.plans.end()) {
+    if (cache.plans.size() >= kMaxBatchPlans) {
+      // the evicted plan's tables may still be read by queued work
+      check_hip(hipStreamSynchronize(nullptr));
+      release_plan(cache.plans.back());
+      cache.plans.pop_back();
+    }
+    std::vector<FastArgs<T>> args(n);
+    std::vector<int2> zmap;
+    zmap.reserve(blocks);
+    for (size_t t = 0; t < n; ++t) {
+      FastArgs<T> f = ex[t]->fast_args();
+      f.in = din[t];
+      f.out = dout[t];
+      f.useScale = static_cast<int>(key[3 + 4 * t + 3]);
+      f.pitchA = fast_xy_pitch_a<T>(m0.dimX, f.ncol);
+      const int b = (f.nst + S - 1) / S;
+      for (int k = 0; k < b; ++k) zmap.push_back(make_int2(static_cast<int>(t), k));
+      args[t] = f;
+    }
+    BatchPlan plan;
+    plan.key = std::move(key);
+    check_hip(hipMalloc(&plan.args, n * sizeof(FastArgs<T>)));
+    check_hip(hipMalloc(&plan.map, zmap.size() * sizeof(int2)));
+    check_hip(hipMemcpyAsync(plan.args, args.data(), n * sizeof(FastArgs<T>),
+                             hipMemcpyHostToDevice, nullptr));
+    check_hip(hipMemcpyAsync(plan.map, zmap.data(), zmap.size() * sizeof(int2),
+                             hipMemcpyHostToDevice, nullptr));
+    cache.plans.push_front(std::move(plan));
+    hit = cache.plans.begin();
+  } else if (hit != cache.plans.begin()) {
+    cache.plans.splice(cache.plans.begin(), cache.plans, hit);
+    hit = cache.plans.begin();
   }
-  check_hip(hipFreeAsync(dArgs, nullptr));
-  check_hip(hipFreeAsync(dMap, nullptr));
+  BatchPlan& plan = *hit;
+  const auto* dArgs = static_cast<const FastArgs<T>*>(plan.args);
+  const auto* dMap = static_cast<const int2*>(plan.map);
+  if (!graphs_enabled() || prof::enabled()) {
+    launch_chain(dArgs, dMap, nullptr);
+    return true;
+  }
+  if (!plan.exec) {
+    // Capture the two launches once on a private stream; later calls with
+    // the same key replay the graph on the default stream.
+    hipStream_t cs = nullptr;
+    check_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+    try {
+      launch_chain(dArgs, dMap, cs);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(cs, &g);
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipStreamDestroy(cs);
+      throw;
+    }
+    check_hip(hipStreamEndCapture(cs, &plan.graph));
+    check_hip(hipStreamDestroy(cs));
+    check_hip(hipGraphInstantiate(&plan.exec, plan.graph, nullptr, nullptr, 0));
+  }
+  check_hip(hipGraphLaunch(plan.exec, nullptr));
   return true;
 }
 

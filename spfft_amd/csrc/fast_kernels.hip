@@ -35,11 +35,13 @@ constexpr int kWave = 64;
 // ---- line shapes and radix schedules (compile time)
 
 // Elements per lane: 8 for powers of two, 24 for lengths with a factor 3.
+// The z kernels use half of that (more lanes per line, more waves in flight).
 constexpr int elems_per_lane(int n) { return n % 3 == 0 ? 24 : 8; }
+constexpr int z_elems_per_lane(int n) { return n % 3 == 0 ? 12 : 4; }
 
-template <int N>
+template <int N, int EL = elems_per_lane(N)>
 struct Line {
-  static constexpr int E = elems_per_lane(N);
+  static constexpr int E = EL;
   static constexpr int P = N / E;      // lanes per line
   static constexpr int PER_WAVE = kWave / P;
   static_assert(N % E == 0 && kWave % P == 0, "line shape");
@@ -70,6 +72,14 @@ constexpr int padded_len(int n) {
 constexpr int make_odd(int n) { return n | 1; }
 constexpr int round_up(int n, int m) { return (n + m - 1) / m * m; }
 
+// Element `i` of an array smaller than 4 GiB, addressed with a 32-bit byte
+// offset (lets the compiler use a scalar base and one offset register).
+template <typename X>
+__device__ __forceinline__ X at32(const X* base, int i) {
+  const unsigned off = static_cast<unsigned>(i) * static_cast<unsigned>(sizeof(X));
+  return *reinterpret_cast<const X*>(reinterpret_cast<const char*>(base) + off);
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -80,10 +90,10 @@ __device__ __forceinline__ void wave_sync() {
 // FFT. `line` is this line's exchange region: element i at line[phys(i) * es]
 // (rows: es = 1; columns of a tile: es = row length). The twiddle table holds
 // W_{N * TWS}. NS = product of the radices already applied.
-template <typename T, int N, bool BWD, int TWS, int NS = 1>
+template <typename T, int N, bool BWD, int TWS, int E = elems_per_lane(N), int NS = 1>
 __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
                                          const Cx<T>* __restrict__ tw) {
-  constexpr int E = Line<N>::E, PL = Line<N>::P;
+  constexpr int PL = N / E;
   constexpr int R = wave_radix(N / NS, E);
   static_assert(R > 0, "no radix divides the lane's element count");
   constexpr int EB = E / R;
@@ -121,7 +131,7 @@ __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
       v[e] = line[phys<T>(i) * es];
     }
     wave_sync();
-    wave_fft<T, N, BWD, TWS, NS * R>(v, line, es, q, tw);
+    wave_fft<T, N, BWD, TWS, E, NS * R>(v, line, es, q, tw);
   }
 }
 
@@ -138,34 +148,56 @@ __device__ __forceinline__ void plane_class(int b, int& z, int& c) {
 
 template <typename T, int NZ>
 struct ZShape {
+  using L = Line<NZ, z_elems_per_lane(NZ)>;
   static constexpr int NT = 256;
-  static constexpr int S = (NT / kWave) * Line<NZ>::PER_WAVE;  // sticks per workgroup
+  static constexpr int S = (NT / kWave) * L::PER_WAVE;  // sticks per workgroup
   static constexpr int LZ = padded_len<T>(NZ);
-  // one spare element after the tile absorbs repeated triplets
-  static size_t lds() { return (static_cast<size_t>(S) * LZ + 1) * sizeof(Cx<T>); }
+  // entries per thread when no triplet repeats (S full sticks)
+  static constexpr int K = (S * NZ + NT - 1) / NT;
+  // tile, one spare element that absorbs repeated triplets, twiddle table
+  static constexpr int TW = S * LZ + 1;
+  static size_t lds() { return (static_cast<size_t>(TW) + NZ) * sizeof(Cx<T>); }
 };
 
 template <typename T, int NZ, bool RC>
 __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
   using Z = ZShape<T, NZ>;
-  using L = Line<NZ>;
-  constexpr int S = Z::S, LZ = Z::LZ, NT = Z::NT;
+  using L = typename Z::L;
+  constexpr int S = Z::S, LZ = Z::LZ, NT = Z::NT, K = Z::K;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
+  Cx<T>* twl = tile + Z::TW;
   const int s0 = bid * S;
   const int ns = min(S, a.nst - s0);
-  for (int t = threadIdx.x; t < S * LZ; t += NT) tile[t] = Cx<T>{T(0), T(0)};
-  __syncthreads();
-  // Entries carry their tile position (a repeated triplet's earlier copies
-  // point at the spare slot after the tile, so the last one wins).
   const int e0 = a.stickBegin[s0], e1 = a.stickBegin[s0 + ns];
-  for (int e = e0 + threadIdx.x; e < e1; e += 2 * NT) {
-    const bool two = e + NT < e1;
-    const int2 ea = a.zEntries[e];
-    const int2 eb = a.zEntries[two ? e + NT : e];
-    const Cx<T> va = reinterpret_cast<const Cx<T>*>(a.in)[ea.x];
-    const Cx<T> vb = reinterpret_cast<const Cx<T>*>(a.in)[eb.x];
-    tile[ea.y < 0 ? S * LZ : ea.y] = va;
-    if (two) tile[eb.y < 0 ? S * LZ : eb.y] = vb;
+  // Entries carry their tile position (a repeated triplet's earlier copies
+  // point at the spare slot after the tile, so the last one wins). All of a
+  // thread's entry loads are issued together, then all its value loads.
+  int ev[K], ep[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int e = e0 + threadIdx.x + k * NT;
+    const int2 x = a.zEntries[e < e1 ? e : e0];
+    ev[k] = x.x;
+    ep[k] = e < e1 && x.y >= 0 ? x.y : S * LZ;
+  }
+  static_assert(NZ <= NT, "one twiddle per thread");
+  const Cx<T> tw = a.twZ[threadIdx.x < NZ ? threadIdx.x : 0];
+  for (int t = threadIdx.x; t < S * LZ; t += NT) tile[t] = Cx<T>{T(0), T(0)};
+  T vr[K], vi[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const Cx<T> x = reinterpret_cast<const Cx<T>*>(a.in)[ev[k]];
+    vr[k] = x.re;
+    vi[k] = x.im;
+  }
+  if (threadIdx.x < NZ) twl[threadIdx.x] = tw;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) tile[ep[k]] = Cx<T>{vr[k], vi[k]};
+  // more entries than S full sticks only with repeated triplets
+  for (int e = e0 + threadIdx.x + K * NT; e < e1; e += NT) {
+    const int2 x = a.zEntries[e];
+    tile[x.y < 0 ? S * LZ : x.y] = reinterpret_cast<const Cx<T>*>(a.in)[x.x];
   }
   __syncthreads();
   if (RC && s0 == 0 && a.zeroZeroStick == 0) {
@@ -185,6 +217,10 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
     }
     __syncthreads();
   }
+  // the xy kernels read element nst of every row as zero
+  if (bid == 0 && threadIdx.x < NZ) {
+    a.zbuf[static_cast<size_t>(threadIdx.x) * a.pitch + a.nst] = Cx<T>{T(0), T(0)};
+  }
   const int lane = threadIdx.x % kWave;
   const int sl = (threadIdx.x / kWave) * L::PER_WAVE + lane / L::P;
   const int q = lane % L::P;
@@ -193,7 +229,7 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
 #pragma unroll
   for (int e = 0; e < L::E; ++e) v[e] = line[phys<T>(q + L::P * e)];
   wave_sync();
-  wave_fft<T, NZ, true, 1>(v, line, 1, q, a.twZ);
+  wave_fft<T, NZ, true, 1, L::E>(v, line, 1, q, twl);
 #pragma unroll
   for (int e = 0; e < L::E; ++e) line[phys<T>(q + L::P * e)] = v[e];
   __syncthreads();
@@ -206,15 +242,27 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
 template <typename T, int NZ>
 __device__ __forceinline__ void z_forward_body(const FastArgs<T>& a, int bid) {
   using Z = ZShape<T, NZ>;
-  using L = Line<NZ>;
-  constexpr int S = Z::S, LZ = Z::LZ, NT = Z::NT;
+  using L = typename Z::L;
+  constexpr int S = Z::S, LZ = Z::LZ, NT = Z::NT, K = Z::K;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
+  Cx<T>* twl = tile + Z::TW;
   const int s0 = bid * S;
   const int ns = min(S, a.nst - s0);
   for (int t = threadIdx.x; t < S * NZ; t += NT) {
     const int s = t % S, zz = t / S;
     tile[s * LZ + phys<T>(zz)] = s < ns ? a.zbuf[static_cast<size_t>(zz) * a.pitch + s0 + s]
                                         : Cx<T>{T(0), T(0)};
+  }
+  if (threadIdx.x < NZ) twl[threadIdx.x] = a.twZ[threadIdx.x];
+  // the gather's entries are loaded while the sticks arrive
+  const int e0 = a.stickBegin[s0], e1 = a.stickBegin[s0 + ns];
+  int ev[K], ep[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int e = e0 + threadIdx.x + k * NT;
+    const int2 x = a.zEntries[e < e1 ? e : e0];
+    ev[k] = e < e1 ? x.x : -1;
+    ep[k] = x.y & 0x7fffffff;
   }
   __syncthreads();
   const int lane = threadIdx.x % kWave;
@@ -225,16 +273,22 @@ __device__ __forceinline__ void z_forward_body(const FastArgs<T>& a, int bid) {
 #pragma unroll
   for (int e = 0; e < L::E; ++e) v[e] = line[phys<T>(q + L::P * e)];
   wave_sync();
-  wave_fft<T, NZ, false, 1>(v, line, 1, q, a.twZ);
+  wave_fft<T, NZ, false, 1, L::E>(v, line, 1, q, twl);
 #pragma unroll
   for (int e = 0; e < L::E; ++e) line[phys<T>(q + L::P * e)] = v[e];
   __syncthreads();
-  const int e0 = a.stickBegin[s0], e1 = a.stickBegin[s0 + ns];
-  for (int e = e0 + threadIdx.x; e < e1; e += NT) {
-    const int2 en = a.zEntries[e];
-    Cx<T> c = tile[en.y & 0x7fffffff];
+  Cx<T>* out = reinterpret_cast<Cx<T>*>(a.out);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    Cx<T> c = tile[ep[k]];
     if (a.useScale) c = cscale(c, a.scale);
-    reinterpret_cast<Cx<T>*>(a.out)[en.x] = c;
+    if (ev[k] >= 0) out[ev[k]] = c;
+  }
+  for (int e = e0 + threadIdx.x + K * NT; e < e1; e += NT) {
+    const int2 x = a.zEntries[e];
+    Cx<T> c = tile[x.y & 0x7fffffff];
+    if (a.useScale) c = cscale(c, a.scale);
+    out[x.x] = c;
   }
 }
 
@@ -255,9 +309,12 @@ struct XYShape {
   // forward: Tb = [rows y][LT], columns transformed in place with stride LT;
   // the rows have room for the spare columns of a partial last wave
   static constexpr int LT = make_odd(padded_len<T>(round_up(M, Line<N>::PER_WAVE)));
-  // Row pitch of A: whole waves of columns, and == 4 (mod 16) so that the
-  // lanes of a 16-lane LDS phase hit distinct bank groups.
-  static int pitch_a(int ncol) { return round_up(round_up(ncol, Line<M>::PER_WAVE) - 4, 16) + 4; }
+  // Row pitch of A: whole waves of columns including the zero column ncol,
+  // and == 4 (mod 16) so that the lanes of a 16-lane LDS phase hit distinct
+  // bank groups.
+  static int pitch_a(int ncol) {
+    return round_up(round_up(ncol + 1, Line<M>::PER_WAVE) - 4, 16) + 4;
+  }
   // elements of the A/B region; R2C appends the filled x = 0 column (N)
   static size_t tile_backward(int ncol) {
     const size_t a = static_cast<size_t>(ROWS_A) * pitch_a(ncol);
@@ -306,9 +363,8 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
   if (fillCol) {
     for (int y = threadIdx.x; y < N; y += S::NT) {
       const int yp = y == 0 ? 0 : N - y;
-      const int sa = a.stickAt[y], sb = a.stickAt[yp];
-      const Cx<T> xs = sa >= 0 ? zrow[sa] : Cx<T>{T(0), T(0)};
-      const Cx<T> xp = sb >= 0 ? zrow[sb] : Cx<T>{T(0), T(0)};
+      const Cx<T> xs = zrow[a.stickAt[y]];
+      const Cx<T> xp = zrow[a.stickAt[yp]];
       Cx<T> o = xs;
       if (y != 0) {
         if (yp == y) {
@@ -334,9 +390,12 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
     wl[0] = Cx<T>{T(1), T(0)};
 #pragma unroll
     for (int l = 1; l < R; ++l) wl[l] = twiddle<true>(a.twY, ((r * l) % R) * M);
-    for (int u0 = wave * LM::PER_WAVE; u0 < ncol; u0 += NW * LM::PER_WAVE) {
+    // Columns up to and including ncol, which comes out zero (its stickAt
+    // row points at the zero element) and serves the x pass as the column of
+    // every absent x. Lanes past it read that row too.
+    for (int u0 = wave * LM::PER_WAVE; u0 <= ncol; u0 += NW * LM::PER_WAVE) {
       const int u = u0 + lane / LM::P;
-      const int uc = u < ncol ? u : ncol - 1;
+      const int uc = u < ncol ? u : ncol;
       const int* rel = a.stickAt + static_cast<size_t>(uc) * N;
       const Cx<T>* col = zrow;
       Cx<T> v[LM::E];
@@ -347,25 +406,22 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
 #pragma unroll
         for (int e = 0; e < EC; ++e) {
 #pragma unroll
-          for (int l = 0; l < R; ++l) sv[e][l] = rel[q + LM::P * (e0 + e) + M * l];
+          for (int l = 0; l < R; ++l) sv[e][l] = at32(rel, q + LM::P * (e0 + e) + M * l);
         }
         Cx<T> xv[EC][R];
 #pragma unroll
         for (int e = 0; e < EC; ++e) {
 #pragma unroll
-          for (int l = 0; l < R; ++l) xv[e][l] = col[sv[e][l] >= 0 ? sv[e][l] : 0];
+          for (int l = 0; l < R; ++l) xv[e][l] = at32(col, sv[e][l]);
         }
 #pragma unroll
         for (int e = 0; e < EC; ++e) {
           const int j = q + LM::P * (e0 + e);
           const Cx<T> wj = twiddle<true>(a.twY, r * j);
-          Cx<T> acc = {T(0), T(0)};
+          Cx<T> acc = xv[e][0];
 #pragma unroll
-          for (int l = 0; l < R; ++l) {
-            const Cx<T> t = l ? cmul(xv[e][l], wl[l]) : xv[e][l];
-            if (sv[e][l] >= 0) acc = cadd(acc, t);
-          }
-          v[e0 + e] = (u < ncol) ? cmul(acc, wj) : Cx<T>{T(0), T(0)};
+          for (int l = 1; l < R; ++l) acc = cadd(acc, cmul(xv[e][l], wl[l]));
+          v[e0 + e] = cmul(acc, wj);
         }
       }
       Cx<T>* tcol = A + u;
@@ -394,15 +450,16 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
   }
   __syncthreads();
 
-  // x: row m (y = r + R m) holds column u's value at x = colX[u].
+  // x: row m (y = r + R m) holds column u's value at x = colX[u]; absent x
+  // read the zero column. Rows past M (a partial last wave) are computed
+  // from row M - 1 and not stored.
   const int m = wave * LX::PER_WAVE + lane / LX::P;
   const int mc = m < M ? m : M - 1;
   const int q = lane % LX::P;
   Cx<T> v[LX::E];
 #pragma unroll
   for (int e = 0; e < LX::E; ++e) {
-    Cx<T> x = A[mc * LU + (uu[e] >= 0 ? uu[e] : 0)];
-    x = (m < M && uu[e] >= 0) ? x : Cx<T>{T(0), T(0)};
+    Cx<T> x = A[mc * LU + uu[e]];
     if (RC) {
       // C2R: the imaginary parts of the DC and Nyquist inputs do not
       // contribute to a real signal; the upper half is conjugated
@@ -522,6 +579,39 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
   }
 }
 
+// Arguments read from a device table hold generic pointers, so the compiler
+// would access them with flat instructions (which share the LDS counter and
+// serialise against LDS traffic). Rebasing each pointer on a kernel argument
+// (a global pointer) keeps the address and lets the accesses be global ones.
+template <typename P>
+__device__ __forceinline__ P* rebase(const char* anchor, P* p) {
+  return (P*)(anchor + (reinterpret_cast<const char*>(p) - anchor));
+}
+template <typename T>
+__device__ __forceinline__ FastArgs<T> on_global(FastArgs<T> a, const void* table) {
+  const char* g = static_cast<const char*>(table);
+  a.zbuf = rebase(g, a.zbuf);
+  a.in = rebase(g, a.in);
+  a.out = rebase(g, a.out);
+  a.entryValue = rebase(g, a.entryValue);
+  a.entrySlot = rebase(g, a.entrySlot);
+  a.stickBegin = rebase(g, a.stickBegin);
+  a.stickY = rebase(g, a.stickY);
+  a.stickCol = rebase(g, a.stickCol);
+  a.colX = rebase(g, a.colX);
+  a.colBegin = rebase(g, a.colBegin);
+  a.stickAt = rebase(g, a.stickAt);
+  a.zEntries = rebase(g, a.zEntries);
+  a.xCol = rebase(g, a.xCol);
+  a.resSticks = rebase(g, a.resSticks);
+  a.resBegin = rebase(g, a.resBegin);
+  a.resYM = rebase(g, a.resYM);
+  a.twX = rebase(g, a.twX);
+  a.twY = rebase(g, a.twY);
+  a.twZ = rebase(g, a.twZ);
+  return a;
+}
+
 // ---- entry points: one transform (arguments by value) or a batch of
 // transforms of one shape (argument table in device memory). A batched z
 // launch maps each workgroup to (transform, block) through `map`; a batched
@@ -547,25 +637,25 @@ template <typename T, int NZ, bool RC>
 __global__ void __launch_bounds__((ZShape<T, NZ>::NT))
     k_z_backward_batch(const FastArgs<T>* __restrict__ batch, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
-  z_backward_body<T, NZ, RC>(batch[m.x], m.y);
+  z_backward_body<T, NZ, RC>(on_global(batch[m.x], batch), m.y);
 }
 template <typename T, int NZ>
 __global__ void __launch_bounds__((ZShape<T, NZ>::NT))
     k_z_forward_batch(const FastArgs<T>* __restrict__ batch, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
-  z_forward_body<T, NZ>(batch[m.x], m.y);
+  z_forward_body<T, NZ>(on_global(batch[m.x], batch), m.y);
 }
 template <typename T, int N, bool RC>
 __global__ void __launch_bounds__((XYShape<T, N>::NT))
     k_xy_backward_batch(const FastArgs<T>* __restrict__ batch, int perTransform) {
   const int t = blockIdx.x / perTransform;
-  xy_backward_body<T, N, RC>(batch[t], blockIdx.x - t * perTransform);
+  xy_backward_body<T, N, RC>(on_global(batch[t], batch), blockIdx.x - t * perTransform);
 }
 template <typename T, int N, bool RC>
 __global__ void __launch_bounds__((XYShape<T, N>::NT))
     k_xy_forward_batch(const FastArgs<T>* __restrict__ batch, int perTransform) {
   const int t = blockIdx.x / perTransform;
-  xy_forward_body<T, N, RC>(batch[t], blockIdx.x - t * perTransform);
+  xy_forward_body<T, N, RC>(on_global(batch[t], batch), blockIdx.x - t * perTransform);
 }
 
 // Raise the dynamic LDS limit of a kernel once per process (the attribute

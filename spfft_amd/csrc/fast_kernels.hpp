@@ -31,7 +31,8 @@ namespace spfft {
 template <typename T>
 struct FastArgs {
   int nz = 0, nst = 0, ncol = 0;
-  int pitch = 0;   // row pitch of zbuf in elements (nst rounded up to 128 bytes)
+  int pitch = 0;   // row pitch of zbuf in elements (nst + 1 rounded up to 128 bytes);
+                   // element nst of every row is zero
   int pitchA = 0;  // row pitch of the backward xy kernel's column tile (set at launch)
   int r2c = 0;            // real space domain, hermitian half in x
   int zeroZeroStick = 0;  // 0 when the (x=0, y=0) stick exists
@@ -45,10 +46,11 @@ struct FastArgs {
   const int* stickCol = nullptr;
   const int* colX = nullptr;
   const int* colBegin = nullptr;
-  const int* stickAt = nullptr;       // [ncol][NY]: stick index at (column, y), or -1
+  const int* stickAt = nullptr;       // [ncol + 1][NY]: stick index at (column, y), or nst
+                                      // (the zero element); row ncol is all nst
   const int2* zEntries = nullptr;     // per value (sorted by slot): {value index,
                                       //  LDS tile position | repeated << 31}
-  const int* xCol = nullptr;          // [NX]: column id of x, or -1
+  const int* xCol = nullptr;          // [NX]: column id of x, or ncol (a zero column)
   const int* resSticks = nullptr;     // sticks grouped by colX mod R
   const int* resBegin = nullptr;      // R + 1 offsets into resSticks
   const int* resYM = nullptr;         // per resSticks entry: y << 16 | (x / R)
@@ -67,11 +69,12 @@ constexpr int fast_classes(int n) { return n >= 192 ? 8 : 4; }
 template <typename T>
 void fast_z_geometry(int nz, int* sticksPerBlock, int* lineLength, int* padShift);
 
-// Row pitch of the [z][stick] buffer: nst rounded up to a 128-byte multiple.
+// Row pitch of the [z][stick] buffer: room for nst sticks and one zero
+// element, rounded up to a 128-byte multiple.
 template <typename T>
 inline int fast_pitch(int nst) {
   const int q = static_cast<int>(128 / (2 * sizeof(T)));
-  return (nst + q - 1) / q * q;
+  return (nst + 1 + q - 1) / q * q;
 }
 
 // True when (nx, ny, nz) have fast kernels and their LDS fits for ncol columns.

@@ -64,8 +64,10 @@ void GridInternal<T>::allocate() {
   const size_t gridSize = static_cast<size_t>(maxDimX_) * maxDimY_ * maxDimZ_;
   space_ = DeviceBuffer<T>(2 * gridSize);
   work_ = DeviceBuffer<T>(2 * gridSize);
-  // rows of the [z][stick] buffer are padded to 128 bytes (16 FP32 sticks)
-  const size_t pitch = (static_cast<size_t>(maxNumLocalZColumns_) + 15) / 16 * 16;
+  // rows of the [z][stick] buffer hold one element past the last stick (kept
+  // zero by the fixed-length kernels) and are padded to 128 bytes (16 FP32
+  // sticks)
+  const size_t pitch = (static_cast<size_t>(maxNumLocalZColumns_) + 1 + 15) / 16 * 16;
   sticks_ = DeviceBuffer<T>(2 * static_cast<size_t>(maxDimZ_) * pitch);
 }
 

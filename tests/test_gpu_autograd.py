@@ -98,7 +98,7 @@ def test_sparse_backward_adjoint_identity():
 
 
 @pytest.mark.parametrize("precision", ["double", "float"])
-def test_cfg5_192_gradient_vs_oracle(precision):
+def test_sphere_192_r60_gradient_vs_oracle(precision):
     """The 192^3, r=60 configuration (904,089 coefficients), one band."""
     from spfft_amd.autograd import sparse_energy
     dims, t, tr, c, v = setup(192, 60, precision, seed=31)

@@ -115,11 +115,11 @@ def test_reference_matrix_r2c(dims, centered):
 
 
 SPHERES = [
-    ("cfg1_32_r12_c2c", (32, 32, 32), 12, False),
-    ("cfg2a_128_r40_c2c", (128, 128, 128), 40, False),
-    ("cfg2b_128_r40_r2c", (128, 128, 128), 40, True),
-    ("96_r30_c2c", (96, 96, 96), 30, False),
-    ("192_r60_c2c", (192, 192, 192), 60, False),
+    ("sphere_32_r12_c2c", (32, 32, 32), 12, False),
+    ("sphere_128_r40_c2c", (128, 128, 128), 40, False),
+    ("sphere_128_r40_r2c", (128, 128, 128), 40, True),
+    ("sphere_96_r30_c2c", (96, 96, 96), 30, False),
+    ("sphere_192_r60_c2c", (192, 192, 192), 60, False),
 ]
 
 
@@ -376,3 +376,43 @@ def test_batched_multi_transform_r2c(precision):
         assert orc.rel_l2(got, want) <= tol, k
         f = outs[k].cpu().numpy().astype(np.float64).view(np.complex128)
         assert orc.rel_l2(f, o.forward(want, True)) <= tol, k
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_batched_graph_replays(precision, monkeypatch):
+    """Repeated multi_transform calls on the same transforms and buffers
+    replay a captured HIP graph. Between replays the inputs are rewritten
+    with eager work and the default stream is synchronised; every replay must
+    equal the direct-launch result bit for bit and match the oracle."""
+    n = 64
+    dims = (n, n, n)
+    sets = [sphere_triplets(r, kshift=(0.1 * k, -0.2, 0.3)) for k, r in enumerate((18.0, 20.0, 21.5))]
+    trs = [build(dims, t, False, precision) for t in sets]
+    dt = torch.float64 if precision == "double" else torch.float32
+    rd = rdt(precision)
+    ins = [torch.empty(2 * len(t), dtype=dt, device="cuda") for t in sets]
+    spaces = [torch.empty(2 * n ** 3, dtype=dt, device="cuda") for _ in trs]
+    outs = [torch.empty(2 * len(t), dtype=dt, device="cuda") for t in sets]
+    tol = FP64_TOL if precision == "double" else FP32_TOL
+    for rep in range(4):
+        vals = [complex_values(500 + 10 * rep + k, len(t)) for k, t in enumerate(sets)]
+        for k, v in enumerate(vals):
+            ins[k].copy_(dev(v.view(np.float64).astype(rd)))
+        torch.cuda.synchronize()
+        results = []
+        for graphs in ("1", "0"):
+            monkeypatch.setenv("SPFFT_AMD_GRAPHS", graphs)
+            sp.multi_transform_backward(trs, ins, spaces)
+            sp.multi_transform_forward(trs, spaces, outs, [sp.SPFFT_FULL_SCALING] * len(trs))
+            torch.cuda.synchronize()
+            results.append(([s.cpu().numpy().copy() for s in spaces], [o.cpu().numpy().copy() for o in outs]))
+        for k in range(len(trs)):
+            np.testing.assert_array_equal(results[0][0][k], results[1][0][k])
+            np.testing.assert_array_equal(results[0][1][k], results[1][1][k])
+        for k, t in enumerate(sets):
+            o = orc.OraclePlan(False, dims, t)
+            want = o.backward(vals[k])
+            got = results[0][0][k].astype(np.float64).view(np.complex128).reshape(n, n, n)
+            assert orc.rel_l2(got, want) <= tol, (rep, k)
+            f = results[0][1][k].astype(np.float64).view(np.complex128)
+            assert orc.rel_l2(f, o.forward(want, True)) <= tol, (rep, k)

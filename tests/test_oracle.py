@@ -167,7 +167,7 @@ def test_delta_gives_plane_wave():
     assert np.max(np.abs(got - want)) < 1e-14
 
 
-def test_cfg1_sphere_round_trip_and_definition():
+def test_sphere_32_r12_round_trip_and_definition():
     dims = (32, 32, 32)
     t = sphere_triplets(12)
     v = complex_values(0x5EED0001, len(t))
