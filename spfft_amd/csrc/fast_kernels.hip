@@ -10,6 +10,7 @@
 // bank conflicts. The last pass leaves the line in natural order in the
 // lanes' registers. Exchanges are wave-local (no workgroup barrier), so the
 // waves of a workgroup run their FFTs independently.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -300,6 +301,11 @@ struct XYShape {
   static constexpr int M = N / R;               // rows (bwd) / columns (fwd) per class
   static constexpr int NT = N >= 128 && N < 192 ? 512 : 256;
   static constexpr int NW = NT / kWave;
+  static_assert(N <= NT, "one twiddle per thread");
+  // lines of M points (backward y, forward x): half the usual elements per
+  // lane, so that a lane's gathers form one batch and twice as many lanes
+  // share them
+  using YL = Line<M, elems_per_lane(M) / 2>;
   // backward: A = [m][LU] (column u at offset u, y transforms in column
   // mode), reused as B = [rows][LB] (x transforms)
   static constexpr int ROWS_A = padded_len<T>(M);
@@ -313,29 +319,30 @@ struct XYShape {
   // and == 4 (mod 16) so that the lanes of a 16-lane LDS phase hit distinct
   // bank groups.
   static int pitch_a(int ncol) {
-    return round_up(round_up(ncol + 1, Line<M>::PER_WAVE) - 4, 16) + 4;
+    return round_up(round_up(ncol + 1, YL::PER_WAVE) - 4, 16) + 4;
   }
-  // elements of the A/B region; R2C appends the filled x = 0 column (N)
+  // elements of the A/B region, followed by the twiddle table (N) and, for
+  // R2C, the filled x = 0 column (N)
   static size_t tile_backward(int ncol) {
     const size_t a = static_cast<size_t>(ROWS_A) * pitch_a(ncol);
     const size_t b = static_cast<size_t>(ROWS_ALLOC) * LB;
     return a > b ? a : b;
   }
   static size_t lds_backward(int ncol, bool rc) {
-    return (tile_backward(ncol) + (rc ? N : 0)) * sizeof(Cx<T>);
+    return (tile_backward(ncol) + N + (rc ? N : 0)) * sizeof(Cx<T>);
   }
   // rows of the tile: the y lines' exchange (padded index map) reaches row
   // padded_len(N) - 1; the x pass may round the row count up to whole waves
   static constexpr int ROWS_FWD =
-      padded_len<T>(N) > round_up(N, Line<M>::PER_WAVE) ? padded_len<T>(N)
-                                                       : round_up(N, Line<M>::PER_WAVE);
+      padded_len<T>(N) > round_up(N, YL::PER_WAVE) ? padded_len<T>(N)
+                                                  : round_up(N, YL::PER_WAVE);
   static size_t lds_forward() { return static_cast<size_t>(ROWS_FWD) * LT * sizeof(Cx<T>); }
 };
 
 template <typename T, int N, bool RC>
 __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) {
   using S = XYShape<T, N>;
-  using LM = Line<S::M>;
+  using LM = typename S::YL;
   using LX = Line<N>;
   constexpr int R = S::R, M = S::M, LB = S::LB, NW = S::NW;
   const int ncol = a.ncol;
@@ -354,11 +361,15 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
     const int x = lane % LX::P + LX::P * e;
     uu[e] = a.xCol[(RC && 2 * x > N) ? N - x : x];
   }
+  // The twiddle table W_N (x and y have the same length) is staged in LDS
+  // after the tile.
+  const int tileA = S::ROWS_A * LU, tileB = S::ROWS_ALLOC * LB;
+  Cx<T>* twl = A + (tileA > tileB ? tileA : tileB);
+  if (threadIdx.x < N) twl[threadIdx.x] = a.twX[threadIdx.x];
   // R2C: hermitian fill of the plane's x = 0 column before its y transform
   // (reference symmetry_host.hpp:47-58, same rule as the (0,0) stick), into
-  // LDS after the tile; the column's gathers read it from there.
-  const int tileA = S::ROWS_A * LU, tileB = S::ROWS_ALLOC * LB;
-  Cx<T>* fill = A + (tileA > tileB ? tileA : tileB);
+  // LDS after the twiddles; the column's gathers read it from there.
+  Cx<T>* fill = twl + N;
   const bool fillCol = RC && a.colX[0] == 0;
   if (fillCol) {
     for (int y = threadIdx.x; y < N; y += S::NT) {
@@ -377,19 +388,19 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
       }
       fill[y] = o;
     }
-    __syncthreads();
   }
+  __syncthreads();
 
   // y: for each column u, decimation in frequency to the rows y = r + R m
   // (t[j] = W^(r j) sum_l X[j + M l] W_R^(r l)), then the M-point FFT. The
-  // gathers are branch free (absent sticks read entry 0 and are masked), so
-  // a lane's loads are all in flight together.
+  // gathers are branch free (absent sticks read the zero element), so a
+  // lane's loads are all in flight together.
   {
     const int q = lane % LM::P;
     Cx<T> wl[R];
     wl[0] = Cx<T>{T(1), T(0)};
 #pragma unroll
-    for (int l = 1; l < R; ++l) wl[l] = twiddle<true>(a.twY, ((r * l) % R) * M);
+    for (int l = 1; l < R; ++l) wl[l] = twiddle<true>(twl, ((r * l) % R) * M);
     // Columns up to and including ncol, which comes out zero (its stickAt
     // row points at the zero element) and serves the x pass as the column of
     // every absent x. Lanes past it read that row too.
@@ -417,7 +428,7 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
 #pragma unroll
         for (int e = 0; e < EC; ++e) {
           const int j = q + LM::P * (e0 + e);
-          const Cx<T> wj = twiddle<true>(a.twY, r * j);
+          const Cx<T> wj = twiddle<true>(twl, r * j);
           Cx<T> acc = xv[e][0];
 #pragma unroll
           for (int l = 1; l < R; ++l) acc = cadd(acc, cmul(xv[e][l], wl[l]));
@@ -425,7 +436,7 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
         }
       }
       Cx<T>* tcol = A + u;
-      wave_fft<T, M, true, R>(v, tcol, LU, q, a.twY);
+      wave_fft<T, M, true, R, LM::E>(v, tcol, LU, q, twl);
 #pragma unroll
       for (int e = 0; e < LM::E; ++e) tcol[(q + LM::P * e) * LU] = v[e];
     }
@@ -440,10 +451,10 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
         Cx<T> acc = fill[j];
 #pragma unroll
         for (int l = 1; l < R; ++l) acc = cadd(acc, cmul(fill[j + M * l], wl[l]));
-        v[e] = cmul(acc, twiddle<true>(a.twY, r * j));
+        v[e] = cmul(acc, twiddle<true>(twl, r * j));
       }
       wave_sync();
-      wave_fft<T, M, true, R>(v, A, LU, q, a.twY);
+      wave_fft<T, M, true, R, LM::E>(v, A, LU, q, twl);
 #pragma unroll
       for (int e = 0; e < LM::E; ++e) A[(q + LM::P * e) * LU] = v[e];
     }
@@ -472,7 +483,7 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
   __syncthreads();  // A is dead from here on; its space holds the row lines
   if (wave * LX::PER_WAVE < M) {
     Cx<T>* line = A + m * LB;
-    wave_fft<T, N, true, 1>(v, line, 1, q, a.twX);
+    wave_fft<T, N, true, 1>(v, line, 1, q, twl);
     if (m < M) {
       const size_t row = (static_cast<size_t>(z) * N + r + R * m) * N;
       if (RC) {
@@ -490,7 +501,7 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
 template <typename T, int N, bool RC>
 __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
   using S = XYShape<T, N>;
-  using LM = Line<S::M>;
+  using LM = typename S::YL;  // x lines of MX points, half-width like the backward y lines
   using LY = Line<N>;
   constexpr int R = S::R, MX = S::M, LT = S::LT, NW = S::NW;
   Cx<T>* Tb = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
@@ -499,6 +510,9 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
   if (z >= a.nz) return;
   const Cx<T>* in = reinterpret_cast<const Cx<T>*>(a.in) + static_cast<size_t>(z) * N * N;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  // twiddles are read from the table in global memory (L1/L2 resident): the
+  // tile leaves no room for a copy at two workgroups per CU
+  const Cx<T>* twl = a.twX;
 
   // x: every row, decimation in frequency to the columns x = c + R m, then
   // the MX-point FFT; row y's result lands in Tb[y][m]. Class twiddles are
@@ -508,10 +522,10 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
     Cx<T> wl[R];
     wl[0] = Cx<T>{T(1), T(0)};
 #pragma unroll
-    for (int l = 1; l < R; ++l) wl[l] = twiddle<false>(a.twX, ((c * l) % R) * MX);
+    for (int l = 1; l < R; ++l) wl[l] = twiddle<false>(twl, ((c * l) % R) * MX);
     Cx<T> wj[LM::E];
 #pragma unroll
-    for (int e = 0; e < LM::E; ++e) wj[e] = twiddle<false>(a.twX, c * (q + LM::P * e));
+    for (int e = 0; e < LM::E; ++e) wj[e] = twiddle<false>(twl, c * (q + LM::P * e));
     for (int y0 = wave * LM::PER_WAVE; y0 < N; y0 += NW * LM::PER_WAVE) {
       const int y = y0 + lane / LM::P;
       const int yc = y < N ? y : N - 1;
@@ -544,7 +558,7 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
         }
       }
       Cx<T>* line = Tb + y * LT;
-      wave_fft<T, MX, false, R>(v, line, 1, q, a.twX);
+      wave_fft<T, MX, false, R, LM::E>(v, line, 1, q, twl);
 #pragma unroll
       for (int e = 0; e < LM::E; ++e) line[phys<T>(q + LM::P * e)] = v[e];
     }
@@ -562,7 +576,7 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
 #pragma unroll
       for (int e = 0; e < LY::E; ++e) v[e] = col[(q + LY::P * e) * LT];
       wave_sync();
-      wave_fft<T, N, false, 1>(v, col, LT, q, a.twY);
+      wave_fft<T, N, false, 1>(v, col, LT, q, twl);
       if (m < MX) {
 #pragma unroll
         for (int e = 0; e < LY::E; ++e) col[(q + LY::P * e) * LT] = v[e];
@@ -680,8 +694,9 @@ void z_backward_n(const FastArgs<T>& a, hipStream_t stream) {
   using Z = ZShape<T, NZ>;
   auto kernel = a.r2c ? k_z_backward_fast<T, NZ, true> : k_z_backward_fast<T, NZ, false>;
   prepare_lds(kernel, Z::lds());
-  prof::KernelSpan span(stream, a.r2c ? "k_z_backward_r2c" : "k_z_backward_fast");
-  hipLaunchKernelGGL(kernel, dim3((a.nst + Z::S - 1) / Z::S), dim3(Z::NT), Z::lds(), stream, a);
+  prof::KernelSpan span(stream, a.r2c ? "k_z_backward_r2c" : "k_z_backward_fast", true);
+  hipExtLaunchKernelGGL(kernel, dim3((a.nst + Z::S - 1) / Z::S), dim3(Z::NT), Z::lds(), stream,
+                        span.start(), span.stop(), 0, a);
   check_launch();
 }
 
@@ -690,8 +705,9 @@ void z_forward_n(const FastArgs<T>& a, hipStream_t stream) {
   using Z = ZShape<T, NZ>;
   auto kernel = k_z_forward_fast<T, NZ>;
   prepare_lds(kernel, Z::lds());
-  prof::KernelSpan span(stream, "k_z_forward_fast");
-  hipLaunchKernelGGL(kernel, dim3((a.nst + Z::S - 1) / Z::S), dim3(Z::NT), Z::lds(), stream, a);
+  prof::KernelSpan span(stream, "k_z_forward_fast", true);
+  hipExtLaunchKernelGGL(kernel, dim3((a.nst + Z::S - 1) / Z::S), dim3(Z::NT), Z::lds(), stream,
+                        span.start(), span.stop(), 0, a);
   check_launch();
 }
 
@@ -707,8 +723,9 @@ void xy_backward_n(FastArgs<T> a, hipStream_t stream) {
   a.pitchA = S::pitch_a(a.ncol);
   auto kernel = a.r2c ? k_xy_backward_fast<T, N, true> : k_xy_backward_fast<T, N, false>;
   prepare_lds(kernel, lds);
-  prof::KernelSpan span(stream, a.r2c ? "k_xy_backward_r2c" : "k_xy_backward_fast");
-  hipLaunchKernelGGL(kernel, dim3(plane_grid<N>(a.nz)), dim3(S::NT), lds, stream, a);
+  prof::KernelSpan span(stream, a.r2c ? "k_xy_backward_r2c" : "k_xy_backward_fast", true);
+  hipExtLaunchKernelGGL(kernel, dim3(plane_grid<N>(a.nz)), dim3(S::NT), lds, stream,
+                        span.start(), span.stop(), 0, a);
   check_launch();
 }
 
@@ -718,8 +735,9 @@ void xy_forward_n(const FastArgs<T>& a, hipStream_t stream) {
   const size_t lds = S::lds_forward();
   auto kernel = a.r2c ? k_xy_forward_fast<T, N, true> : k_xy_forward_fast<T, N, false>;
   prepare_lds(kernel, lds);
-  prof::KernelSpan span(stream, a.r2c ? "k_xy_forward_r2c" : "k_xy_forward_fast");
-  hipLaunchKernelGGL(kernel, dim3(plane_grid<N>(a.nz)), dim3(S::NT), lds, stream, a);
+  prof::KernelSpan span(stream, a.r2c ? "k_xy_forward_r2c" : "k_xy_forward_fast", true);
+  hipExtLaunchKernelGGL(kernel, dim3(plane_grid<N>(a.nz)), dim3(S::NT), lds, stream,
+                        span.start(), span.stop(), 0, a);
   check_launch();
 }
 
@@ -750,9 +768,10 @@ void xy_backward_batch_n(bool rc, const FastArgs<T>* batch, int ntrans, int nz, 
   const size_t lds = S::lds_backward(maxNcol, rc);
   auto kernel = rc ? k_xy_backward_batch<T, N, true> : k_xy_backward_batch<T, N, false>;
   prepare_lds(kernel, lds);
-  prof::KernelSpan span(stream, rc ? "k_xy_backward_batch_r2c" : "k_xy_backward_batch");
+  prof::KernelSpan span(stream, rc ? "k_xy_backward_batch_r2c" : "k_xy_backward_batch", true);
   const int per = plane_grid<N>(nz);
-  hipLaunchKernelGGL(kernel, dim3(per * ntrans), dim3(S::NT), lds, stream, batch, per);
+  hipExtLaunchKernelGGL(kernel, dim3(per * ntrans), dim3(S::NT), lds, stream, span.start(),
+                        span.stop(), 0, batch, per);
   check_launch();
 }
 
@@ -763,9 +782,10 @@ void xy_forward_batch_n(bool rc, const FastArgs<T>* batch, int ntrans, int nz,
   const size_t lds = S::lds_forward();
   auto kernel = rc ? k_xy_forward_batch<T, N, true> : k_xy_forward_batch<T, N, false>;
   prepare_lds(kernel, lds);
-  prof::KernelSpan span(stream, rc ? "k_xy_forward_batch_r2c" : "k_xy_forward_batch");
+  prof::KernelSpan span(stream, rc ? "k_xy_forward_batch_r2c" : "k_xy_forward_batch", true);
   const int per = plane_grid<N>(nz);
-  hipLaunchKernelGGL(kernel, dim3(per * ntrans), dim3(S::NT), lds, stream, batch, per);
+  hipExtLaunchKernelGGL(kernel, dim3(per * ntrans), dim3(S::NT), lds, stream, span.start(),
+                        span.stop(), 0, batch, per);
   check_launch();
 }
 
@@ -776,13 +796,15 @@ void z_batch_n(bool forward, bool rc, const FastArgs<T>* batch, const int2* map,
   if (forward) {
     auto kernel = k_z_forward_batch<T, NZ>;
     prepare_lds(kernel, Z::lds());
-    prof::KernelSpan span(stream, "k_z_forward_batch");
-    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(Z::NT), Z::lds(), stream, batch, map);
+    prof::KernelSpan span(stream, "k_z_forward_batch", true);
+    hipExtLaunchKernelGGL(kernel, dim3(blocks), dim3(Z::NT), Z::lds(), stream,
+                          span.start(), span.stop(), 0, batch, map);
   } else {
     auto kernel = rc ? k_z_backward_batch<T, NZ, true> : k_z_backward_batch<T, NZ, false>;
     prepare_lds(kernel, Z::lds());
-    prof::KernelSpan span(stream, rc ? "k_z_backward_batch_r2c" : "k_z_backward_batch");
-    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(Z::NT), Z::lds(), stream, batch, map);
+    prof::KernelSpan span(stream, rc ? "k_z_backward_batch_r2c" : "k_z_backward_batch", true);
+    hipExtLaunchKernelGGL(kernel, dim3(blocks), dim3(Z::NT), Z::lds(), stream,
+                          span.start(), span.stop(), 0, batch, map);
   }
   check_launch();
 }

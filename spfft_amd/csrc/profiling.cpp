@@ -40,18 +40,24 @@ hipEvent_t take_event() {
 
 bool enabled() { return g_enabled.load(std::memory_order_relaxed); }
 
-KernelSpan::KernelSpan(hipStream_t stream, const char* name) : stream_(stream) {
+KernelSpan::KernelSpan(hipStream_t stream, const char* name, bool attach)
+    : stream_(stream), attach_(attach) {
   if (!enabled()) return;
   std::lock_guard<std::mutex> lock(g_mutex);
   Record r{name, take_event(), take_event()};
   if (!r.start || !r.end) return;
-  (void)hipEventRecord(r.start, stream_);
+  if (attach_) {
+    start_ = r.start;
+    stop_ = r.end;
+  } else {
+    (void)hipEventRecord(r.start, stream_);
+  }
   slot_ = static_cast<int>(g_records.size());
   g_records.push_back(r);
 }
 
 KernelSpan::~KernelSpan() {
-  if (slot_ < 0) return;
+  if (slot_ < 0 || attach_) return;
   std::lock_guard<std::mutex> lock(g_mutex);
   (void)hipEventRecord(g_records[slot_].end, stream_);
 }
