@@ -8,6 +8,9 @@ Matrix of the reference's local test (tests/local_tests/test_local_transform.cpp
 sizes {1, 2, 11, 12, 13, 100} per dimension, centred and non-centred,
 C2C backward, C2C forward, R2C forward+backward.
 """
+import glob
+import os
+
 import numpy as np
 import pytest
 
@@ -416,3 +419,25 @@ def test_batched_graph_replays(precision, monkeypatch):
             assert orc.rel_l2(got, want) <= tol, (rep, k)
             f = results[0][1][k].astype(np.float64).view(np.complex128)
             assert orc.rel_l2(f, o.forward(want, True)) <= tol, (rep, k)
+
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("path", GOLDEN, ids=lambda p: os.path.basename(p)[:-4])
+def test_golden_fixtures(path, precision):
+    """The HIP library against the pocketfft fixtures (tests/golden/make_fixtures.py)."""
+    g = np.load(path)
+    dims = tuple(int(x) for x in g["dims"])
+    r2c = bool(g["r2c"])
+    tol = FP64_TOL if precision == "double" else FP32_TOL
+    tr = build(dims, g["triplets"], r2c, precision)
+    b = gpu_backward(tr, g["values"], dims, r2c, precision)
+    flat = b.ravel()
+    if "backward" in g:
+        assert orc.rel_l2(flat, g["backward"]) <= tol
+    else:
+        assert orc.rel_l2(flat[g["backward_index"]], g["backward_sample"]) <= tol
+    f = gpu_forward(tr, b, r2c, sp.SPFFT_FULL_SCALING, precision)
+    assert orc.rel_l2(f, g["forward_scaled"]) <= tol

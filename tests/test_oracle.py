@@ -13,6 +13,9 @@ The reference ships no golden vectors (tests generate data at run time and
 compare with FFTW 3D at 1e-6 absolute, tests/test_util/test_check_values.hpp);
 the bounds here are tighter.
 """
+import glob
+import os
+
 import numpy as np
 import pytest
 
@@ -250,3 +253,23 @@ def test_r2c_upper_half_fills_missing_lower():
     got = orc.OraclePlan(True, dims, t).backward(v)
     want = orc.dense_backward_r2c_hermitian(dims, t, v)
     assert orc.rel_l2(got, want) < 1e-14
+
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=lambda p: os.path.basename(p)[:-4])
+def test_oracle_matches_golden_fixtures(path):
+    """The C oracle against the pocketfft fixtures (tests/golden/make_fixtures.py)."""
+    g = np.load(path)
+    dims = tuple(int(x) for x in g["dims"])
+    r2c = bool(g["r2c"])
+    o = orc.OraclePlan(r2c, dims, g["triplets"])
+    b = o.backward(g["values"]).ravel()
+    if "backward" in g:
+        assert orc.rel_l2(b, g["backward"]) <= 1e-13
+    else:
+        assert orc.rel_l2(b[g["backward_index"]], g["backward_sample"]) <= 1e-13
+        assert abs(np.linalg.norm(b) - float(g["backward_norm"])) <= 1e-12 * float(g["backward_norm"])
+    space = b.reshape(dims[2], dims[1], dims[0])
+    assert orc.rel_l2(o.forward(space, True), g["forward_scaled"]) <= 1e-13
