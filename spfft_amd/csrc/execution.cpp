@@ -125,10 +125,16 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
     // Stick index at (column, y); absent positions, and the extra row
     // `numCols_` used by the lanes past the last column, point at the zero
     // element that every row of the [z][stick] buffer holds at index nst.
+    // Stored class-interleaved: for column u and j < M = Ny / R the R entries
+    // of y = j + M l (l < R) are adjacent, so a lane of the backward plane
+    // kernel reads all the indices of one decimation-in-frequency output with
+    // R / 4 16-byte loads.
     const int ny = map_->dimY;
+    const int R = fast_classes(map_->dimX), M = ny / R;
     std::vector<int> at(static_cast<size_t>(numCols_ + 1) * ny, map_->num_sticks());
     for (int s = 0; s < map_->num_sticks(); ++s) {
-      at[static_cast<size_t>(t.stickCol[s]) * ny + t.stickY[s]] = s;
+      const int y = t.stickY[s];
+      at[(static_cast<size_t>(t.stickCol[s]) * M + y % M) * R + y / M] = s;
     }
     stickAt_ = DeviceBuffer<int>::from_host(at);
     // Entry list for the z kernels: value index and position in the LDS tile

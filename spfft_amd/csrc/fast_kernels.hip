@@ -136,6 +136,27 @@ __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
   }
 }
 
+// One output of the decimation-in-frequency split over R inputs x[l] with
+// the class's twiddles w[l] = W_R^(+-r l): sum_l w[l] x[l]. For R = 4 the
+// twiddles are 1, w1, w2 = +-1, w3 = conj(w1), and the sum needs 10 real
+// operations instead of 18: w1 x1 + conj(w1) x3 = (c p.re - s d.im,
+// c p.im + s d.re) with p = x1 + x3, d = x1 - x3, w1 = c + i s.
+template <typename T, int R>
+__device__ __forceinline__ Cx<T> dif_sum(const Cx<T>* x, const Cx<T>* w) {
+  if constexpr (R == 4) {
+    const T w2 = w[2].re;
+    const T pr = x[1].re + x[3].re, pi = x[1].im + x[3].im;
+    const T dr = x[1].re - x[3].re, di = x[1].im - x[3].im;
+    return Cx<T>{x[0].re + w2 * x[2].re + w[1].re * pr - w[1].im * di,
+                 x[0].im + w2 * x[2].im + w[1].re * pi + w[1].im * dr};
+  } else {
+    Cx<T> acc = x[0];
+#pragma unroll
+    for (int l = 1; l < R; ++l) acc = cadd(acc, cmul(x[l], w[l]));
+    return acc;
+  }
+}
+
 // Workgroup id -> (plane, class). Workgroups are dealt to the 8 XCDs round
 // robin, so the classes of a plane get ids with the same id mod 8 and share
 // that XCD's L2 for the plane's input.
@@ -374,8 +395,8 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
   if (fillCol) {
     for (int y = threadIdx.x; y < N; y += S::NT) {
       const int yp = y == 0 ? 0 : N - y;
-      const Cx<T> xs = zrow[a.stickAt[y]];
-      const Cx<T> xp = zrow[a.stickAt[yp]];
+      const Cx<T> xs = zrow[a.stickAt[(y % M) * R + y / M]];
+      const Cx<T> xp = zrow[a.stickAt[(yp % M) * R + yp / M]];
       Cx<T> o = xs;
       if (y != 0) {
         if (yp == y) {
@@ -411,13 +432,21 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
       const Cx<T>* col = zrow;
       Cx<T> v[LM::E];
       constexpr int EC = LM::E % 4 == 0 ? 4 : LM::E;  // elements gathered per batch
+      static_assert(R % 4 == 0, "16-byte index loads");
 #pragma unroll
       for (int e0 = 0; e0 < LM::E; e0 += EC) {
         int sv[EC][R];
 #pragma unroll
         for (int e = 0; e < EC; ++e) {
+          const int j = q + LM::P * (e0 + e);
 #pragma unroll
-          for (int l = 0; l < R; ++l) sv[e][l] = at32(rel, q + LM::P * (e0 + e) + M * l);
+          for (int l = 0; l < R; l += 4) {
+            const int4 x = at32(reinterpret_cast<const int4*>(rel), (j * R + l) / 4);
+            sv[e][l] = x.x;
+            sv[e][l + 1] = x.y;
+            sv[e][l + 2] = x.z;
+            sv[e][l + 3] = x.w;
+          }
         }
         Cx<T> xv[EC][R];
 #pragma unroll
@@ -429,10 +458,7 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
         for (int e = 0; e < EC; ++e) {
           const int j = q + LM::P * (e0 + e);
           const Cx<T> wj = twiddle<true>(twl, r * j);
-          Cx<T> acc = xv[e][0];
-#pragma unroll
-          for (int l = 1; l < R; ++l) acc = cadd(acc, cmul(xv[e][l], wl[l]));
-          v[e0 + e] = cmul(acc, wj);
+          v[e0 + e] = cmul(dif_sum<T, R>(xv[e], wl), wj);
         }
       }
       Cx<T>* tcol = A + u;
@@ -448,10 +474,10 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
 #pragma unroll
       for (int e = 0; e < LM::E; ++e) {
         const int j = q + LM::P * e;
-        Cx<T> acc = fill[j];
+        Cx<T> xf[R];
 #pragma unroll
-        for (int l = 1; l < R; ++l) acc = cadd(acc, cmul(fill[j + M * l], wl[l]));
-        v[e] = cmul(acc, twiddle<true>(twl, r * j));
+        for (int l = 0; l < R; ++l) xf[l] = fill[j + M * l];
+        v[e] = cmul(dif_sum<T, R>(xf, wl), twiddle<true>(twl, r * j));
       }
       wave_sync();
       wave_fft<T, M, true, R, LM::E>(v, A, LU, q, twl);
@@ -551,10 +577,7 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
         }
 #pragma unroll
         for (int e = 0; e < EC; ++e) {
-          Cx<T> acc = xv[e][0];
-#pragma unroll
-          for (int l = 1; l < R; ++l) acc = cadd(acc, cmul(xv[e][l], wl[l]));
-          v[e0 + e] = cmul(acc, wj[e0 + e]);
+          v[e0 + e] = cmul(dif_sum<T, R>(xv[e], wl), wj[e0 + e]);
         }
       }
       Cx<T>* line = Tb + y * LT;

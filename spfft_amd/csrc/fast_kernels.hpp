@@ -46,8 +46,8 @@ struct FastArgs {
   const int* stickCol = nullptr;
   const int* colX = nullptr;
   const int* colBegin = nullptr;
-  const int* stickAt = nullptr;       // [ncol + 1][NY]: stick index at (column, y), or nst
-                                      // (the zero element); row ncol is all nst
+  const int* stickAt = nullptr;       // [ncol + 1][M][R]: stick index at (column, y = j + M l),
+                                      // or nst (the zero element); row ncol is all nst
   const int2* zEntries = nullptr;     // per value (sorted by slot): {value index,
                                       //  LDS tile position | repeated << 31}
   const int* xCol = nullptr;          // [NX]: column id of x, or ncol (a zero column)
