@@ -290,6 +290,7 @@ def main():
               "several GPUs here.", file=sys.stderr)
         sys.exit(2)
 
+    os.environ.setdefault("SPFFT_AMD_TUNE_LOG", "1")  # batched kernel choices go to the log
     import numpy as np
     import torch
 

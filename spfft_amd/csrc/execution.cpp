@@ -4,8 +4,11 @@
 #include <atomic>
 #include <cstdint>
 #include <cstdlib>
+#include <cstdio>
 #include <list>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -42,6 +45,32 @@ bool disable_fast_path() {
   return v && v[0] == '1';
 }
 
+// Choice of plane kernels for batches: SPFFT_AMD_PLANE=0 keeps the
+// class-split kernels, =1 forces the whole-plane kernels where they apply;
+// unset, the autotuner decides. Returns -1 (tune), 0 or 1.
+int plane_kernels_forced() {
+  const char* v = std::getenv("SPFFT_AMD_PLANE");
+  if (v && v[0] == '0') return 0;
+  if (v && v[0] == '1') return 1;
+  return -1;
+}
+
+// Autotuner decisions between the class-split and the whole-plane kernels,
+// per (value bytes, length, direction): timed once, on the first qualifying
+// batch, on that batch's own buffers. SPFFT_AMD_TUNE_LOG=1 prints them.
+struct TuneTable {
+  std::mutex mutex;
+  std::map<std::tuple<int, int, int>, int> choice;
+};
+TuneTable& tune_table() {
+  static TuneTable* t = new TuneTable();
+  return *t;
+}
+bool tune_log() {
+  const char* v = std::getenv("SPFFT_AMD_TUNE_LOG");
+  return v && v[0] == '1';
+}
+
 // SPFFT_AMD_GRAPHS=0 replays batched calls with direct launches instead of
 // captured HIP graphs.
 bool graphs_enabled() {
@@ -61,6 +90,7 @@ struct BatchPlan {
   void* map = nullptr;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
+  bool plane = false;  // plane kernels the graph was captured with
 };
 
 // Most recently used first. Heap-allocated and never destroyed: plans hold
@@ -173,6 +203,13 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
     xCol_ = DeviceBuffer<int>::from_host(xCol);
     resSticks_ = DeviceBuffer<int>::from_host(resSticks);
     resBegin_ = DeviceBuffer<int>::from_host(resBegin);
+    // tile position of every stick for the whole-plane batch kernels
+    if (plane_path_supported<T>(map_->dimX, numCols_, map_->type == SPFFT_TRANS_R2C)) {
+      pitchT_ = plane_pitch<T>(map_->dimX, numCols_);
+      std::vector<int> tpos(map_->num_sticks());
+      for (int s = 0; s < map_->num_sticks(); ++s) tpos[s] = t.stickY[s] * pitchT_ + t.stickCol[s];
+      tpos_ = DeviceBuffer<int>::from_host(tpos);
+    }
   }
 
   twX_ = DeviceBuffer<Cx<T>>(map_->dimX);
@@ -255,6 +292,8 @@ FastArgs<T> ExecutionGPU<T>::fast_args() const {
   a.resSticks = resSticks_.data();
   a.resBegin = resBegin_.data();
   a.xCol = xCol_.data();
+  a.tpos = tpos_.data();
+  a.pitchT = pitchT_;
   a.resYM = resYM_.data();
   a.twX = twX_.data();
   a.twY = twY_.data();
@@ -430,13 +469,28 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
   }
   const bool r2c = m0.type == SPFFT_TRANS_R2C;
   const int nt = static_cast<int>(n);
+  // whole-plane kernels are a candidate when every transform has its tile
+  // positions and the widest tile fits
+  bool planeOk = plane_kernels_forced() != 0 && plane_path_supported<T>(m0.dimX, maxNcol, r2c);
+  for (size_t t = 0; t < n && planeOk; ++t) planeOk = ex[t]->tpos_.data() != nullptr;
+  bool plane = false;
+  auto launch_xy_variant = [&](bool usePlane, const FastArgs<T>* dArgs, hipStream_t s) {
+    if (usePlane) {
+      launch_plane_xy_batch<T>(forward, m0.dimX, dArgs, nt, m0.dimZ, maxNcol, s);
+    } else {
+      launch_fast_xy_batch<T>(forward, r2c, m0.dimX, dArgs, nt, m0.dimZ, maxNcol, s);
+    }
+  };
+  auto launch_xy = [&](const FastArgs<T>* dArgs, hipStream_t s) {
+    launch_xy_variant(plane, dArgs, s);
+  };
   auto launch_chain = [&](const FastArgs<T>* dArgs, const int2* dMap, hipStream_t s) {
     if (forward) {
-      launch_fast_xy_batch<T>(true, r2c, m0.dimX, dArgs, nt, m0.dimZ, maxNcol, s);
+      launch_xy(dArgs, s);
       launch_fast_z_batch<T>(true, r2c, m0.dimZ, dArgs, dMap, blocks, s);
     } else {
       launch_fast_z_batch<T>(false, r2c, m0.dimZ, dArgs, dMap, blocks, s);
-      launch_fast_xy_batch<T>(false, r2c, m0.dimX, dArgs, nt, m0.dimZ, maxNcol, s);
+      launch_xy(dArgs, s);
     }
   };
 
@@ -481,11 +535,59 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
   BatchPlan& plan = *hit;
   const auto* dArgs = static_cast<const FastArgs<T>*>(plan.args);
   const auto* dMap = static_cast<const int2*>(plan.map);
+
+  if (planeOk) {
+    const int forced = plane_kernels_forced();
+    const auto tkey = std::make_tuple(static_cast<int>(sizeof(T)), m0.dimX, forward ? 1 : 0);
+    TuneTable& tt = tune_table();
+    std::lock_guard<std::mutex> tlock(tt.mutex);
+    auto it = tt.choice.find(tkey);
+    if (forced == 1) {
+      plane = true;
+    } else if (it != tt.choice.end()) {
+      plane = it->second != 0;
+    } else {
+      // Time both plane kernels on this batch. The xy launch is idempotent
+      // (backward: [z][stick] -> outputs, after the z launch has filled the
+      // buffer; forward: inputs -> [z][stick]), and the call's real launch
+      // chain follows.
+      if (!forward) launch_fast_z_batch<T>(false, r2c, m0.dimZ, dArgs, dMap, blocks, nullptr);
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      check_hip(hipEventCreate(&e0));
+      check_hip(hipEventCreate(&e1));
+      float ms[2] = {0.f, 0.f};
+      for (int c = 0; c < 2; ++c) {
+        launch_xy_variant(c == 1, dArgs, nullptr);
+        check_hip(hipEventRecord(e0, nullptr));
+        for (int k = 0; k < 3; ++k) launch_xy_variant(c == 1, dArgs, nullptr);
+        check_hip(hipEventRecord(e1, nullptr));
+        check_hip(hipEventSynchronize(e1));
+        check_hip(hipEventElapsedTime(&ms[c], e0, e1));
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      plane = ms[1] < ms[0];
+      tt.choice[tkey] = plane ? 1 : 0;
+      if (tune_log()) {
+        std::fprintf(stderr, "spfft_amd tune: %s N=%d %s, %zu transforms: class-split %.3f ms, "
+                     "whole-plane %.3f ms -> %s\n", sizeof(T) == 8 ? "double" : "float",
+                     m0.dimX, forward ? "forward" : "backward", n, ms[0] / 3, ms[1] / 3,
+                     plane ? "whole-plane" : "class-split");
+      }
+    }
+  }
+  if (plan.exec && plan.plane != plane) {
+    (void)hipGraphExecDestroy(plan.exec);
+    (void)hipGraphDestroy(plan.graph);
+    plan.exec = nullptr;
+    plan.graph = nullptr;
+  }
   if (!graphs_enabled() || prof::enabled()) {
     launch_chain(dArgs, dMap, nullptr);
     return true;
   }
   if (!plan.exec) {
+    plan.plane = plane;
     // Capture the two launches once on a private stream; later calls with
     // the same key replay the graph on the default stream.
     hipStream_t cs = nullptr;

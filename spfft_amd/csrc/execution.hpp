@@ -78,6 +78,9 @@ private:
   DeviceBuffer<int> stickAt_;
   DeviceBuffer<int2> zEntries_;
   DeviceBuffer<int> resSticks_, resBegin_, xCol_, resYM_;
+  // whole-plane batch kernels (power-of-two lengths up to 128, C2C)
+  DeviceBuffer<int> tpos_;
+  int pitchT_ = 0;
 };
 
 }  // namespace spfft

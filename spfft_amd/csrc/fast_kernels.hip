@@ -616,6 +616,143 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
   }
 }
 
+// ------------------------------------------------------ whole-plane kernels
+//
+// One workgroup per plane for batches of transforms: the plane tile
+// T[y][column] (columns of the sparse set plus a zero column) lives in LDS,
+// the y transforms run in column mode on it and the x transforms on rows, so
+// every input element is read from memory once (the class-split kernels read
+// a plane R times). x lines have 4 elements per lane and exchange through a
+// per-wave scratch region after the tile; y lines have 8.
+
+template <typename T, int N>
+struct PlaneShape {
+  static constexpr int NT = N >= 128 ? 1024 : (N >= 64 ? 512 : 256);
+  static constexpr int NW = NT / kWave;
+  using XL = Line<N, 4>;
+  using YL = Line<N, 8>;
+  static constexpr int ROWS = padded_len<T>(N);  // the column exchange reaches padded rows
+  static constexpr int LB = padded_len<T>(N);    // x line in the scratch region
+  static constexpr int XROWS = NW * XL::PER_WAVE;
+  static_assert(N <= NT, "one twiddle per thread");
+  static_assert(N % XROWS == 0, "whole x iterations");
+  static int pitch(int ncol) { return round_up(round_up(ncol + 1, YL::PER_WAVE) - 4, 16) + 4; }
+  static size_t lds(int ncol) {
+    return (static_cast<size_t>(ROWS) * pitch(ncol) + static_cast<size_t>(XROWS) * LB + N) *
+           sizeof(Cx<T>);
+  }
+};
+
+template <typename T, int N>
+__device__ __forceinline__ void plane_backward_body(const FastArgs<T>& a, int z) {
+  using S = PlaneShape<T, N>;
+  using XL = typename S::XL;
+  using YL = typename S::YL;
+  constexpr int NT = S::NT, NW = S::NW, LB = S::LB;
+  const int PU = a.pitchT, ncol = a.ncol;
+  Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
+  Cx<T>* xs = tile + S::ROWS * PU;
+  Cx<T>* twl = xs + S::XROWS * LB;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  if (threadIdx.x < N) twl[threadIdx.x] = a.twX[threadIdx.x];
+  for (int t = threadIdx.x; t < S::ROWS * PU; t += NT) tile[t] = Cx<T>{T(0), T(0)};
+  // column of each x this lane handles in the x pass (ncol = zero column)
+  int uu[XL::E];
+#pragma unroll
+  for (int e = 0; e < XL::E; ++e) uu[e] = a.xCol[lane % XL::P + XL::P * e];
+  __syncthreads();
+  const Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
+  for (int st = threadIdx.x; st < a.nst; st += NT) tile[a.tpos[st]] = zrow[st];
+  __syncthreads();
+  // y: columns in column mode; columns past ncol are zero and stay zero
+  {
+    const int q = lane % YL::P;
+    for (int u0 = wave * YL::PER_WAVE; u0 < ncol; u0 += NW * YL::PER_WAVE) {
+      const int u = u0 + lane / YL::P;
+      Cx<T>* col = tile + u;
+      Cx<T> v[YL::E];
+#pragma unroll
+      for (int e = 0; e < YL::E; ++e) v[e] = col[(q + YL::P * e) * PU];
+      wave_sync();
+      wave_fft<T, N, true, 1, YL::E>(v, col, PU, q, twl);
+#pragma unroll
+      for (int e = 0; e < YL::E; ++e) col[(q + YL::P * e) * PU] = v[e];
+    }
+  }
+  __syncthreads();
+  // x: rows, XROWS at a time, each lane group exchanging through its own line
+  const int q = lane % XL::P;
+  const int lr = wave * XL::PER_WAVE + lane / XL::P;
+  Cx<T>* line = xs + lr * LB;
+  Cx<T>* out = reinterpret_cast<Cx<T>*>(a.out) + static_cast<size_t>(z) * N * N;
+  for (int y0 = 0; y0 < N; y0 += S::XROWS) {
+    const int y = y0 + lr;
+    Cx<T> v[XL::E];
+#pragma unroll
+    for (int e = 0; e < XL::E; ++e) v[e] = tile[y * PU + uu[e]];
+    wave_fft<T, N, true, 1, XL::E>(v, line, 1, q, twl);
+#pragma unroll
+    for (int e = 0; e < XL::E; ++e) out[static_cast<size_t>(y) * N + q + XL::P * e] = v[e];
+    wave_sync();
+  }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) {
+  using S = PlaneShape<T, N>;
+  using XL = typename S::XL;
+  using YL = typename S::YL;
+  constexpr int NT = S::NT, NW = S::NW, LB = S::LB;
+  const int PU = a.pitchT, ncol = a.ncol;
+  Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
+  Cx<T>* xs = tile + S::ROWS * PU;
+  Cx<T>* twl = xs + S::XROWS * LB;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  if (threadIdx.x < N) twl[threadIdx.x] = a.twX[threadIdx.x];
+  int uu[XL::E];
+#pragma unroll
+  for (int e = 0; e < XL::E; ++e) uu[e] = a.xCol[lane % XL::P + XL::P * e];
+  __syncthreads();
+  // x: every row, read once; the sparse set's columns go into the tile
+  {
+    const int q = lane % XL::P;
+    const int lr = wave * XL::PER_WAVE + lane / XL::P;
+    Cx<T>* line = xs + lr * LB;
+    const Cx<T>* in = reinterpret_cast<const Cx<T>*>(a.in) + static_cast<size_t>(z) * N * N;
+    for (int y0 = 0; y0 < N; y0 += S::XROWS) {
+      const int y = y0 + lr;
+      Cx<T> v[XL::E];
+#pragma unroll
+      for (int e = 0; e < XL::E; ++e) v[e] = in[static_cast<size_t>(y) * N + q + XL::P * e];
+      wave_fft<T, N, false, 1, XL::E>(v, line, 1, q, twl);
+#pragma unroll
+      for (int e = 0; e < XL::E; ++e) {
+        if (uu[e] < ncol) tile[y * PU + uu[e]] = v[e];
+      }
+      wave_sync();
+    }
+  }
+  __syncthreads();
+  // y: the sparse set's columns in column mode
+  {
+    const int q = lane % YL::P;
+    for (int u0 = wave * YL::PER_WAVE; u0 < ncol; u0 += NW * YL::PER_WAVE) {
+      const int u = u0 + lane / YL::P;
+      Cx<T>* col = tile + u;
+      Cx<T> v[YL::E];
+#pragma unroll
+      for (int e = 0; e < YL::E; ++e) v[e] = col[(q + YL::P * e) * PU];
+      wave_sync();
+      wave_fft<T, N, false, 1, YL::E>(v, col, PU, q, twl);
+#pragma unroll
+      for (int e = 0; e < YL::E; ++e) col[(q + YL::P * e) * PU] = v[e];
+    }
+  }
+  __syncthreads();
+  Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
+  for (int st = threadIdx.x; st < a.nst; st += NT) zrow[st] = tile[a.tpos[st]];
+}
+
 // Arguments read from a device table hold generic pointers, so the compiler
 // would access them with flat instructions (which share the LDS counter and
 // serialise against LDS traffic). Rebasing each pointer on a kernel argument
@@ -643,6 +780,7 @@ __device__ __forceinline__ FastArgs<T> on_global(FastArgs<T> a, const void* tabl
   a.resSticks = rebase(g, a.resSticks);
   a.resBegin = rebase(g, a.resBegin);
   a.resYM = rebase(g, a.resYM);
+  a.tpos = rebase(g, a.tpos);
   a.twX = rebase(g, a.twX);
   a.twY = rebase(g, a.twY);
   a.twZ = rebase(g, a.twZ);
@@ -693,6 +831,19 @@ __global__ void __launch_bounds__((XYShape<T, N>::NT))
     k_xy_forward_batch(const FastArgs<T>* __restrict__ batch, int perTransform) {
   const int t = blockIdx.x / perTransform;
   xy_forward_body<T, N, RC>(on_global(batch[t], batch), blockIdx.x - t * perTransform);
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__((PlaneShape<T, N>::NT))
+    k_plane_backward_batch(const FastArgs<T>* __restrict__ batch, int nz) {
+  const int t = blockIdx.x / nz;
+  plane_backward_body<T, N>(on_global(batch[t], batch), blockIdx.x - t * nz);
+}
+template <typename T, int N>
+__global__ void __launch_bounds__((PlaneShape<T, N>::NT))
+    k_plane_forward_batch(const FastArgs<T>* __restrict__ batch, int nz) {
+  const int t = blockIdx.x / nz;
+  plane_forward_body<T, N>(on_global(batch[t], batch), blockIdx.x - t * nz);
 }
 
 // Raise the dynamic LDS limit of a kernel once per process (the attribute
@@ -913,7 +1064,60 @@ void launch_fast_xy_batch(bool forward, bool r2c, int n, const FastArgs<T>* batc
   }
 }
 
+template <typename T, int N>
+void plane_batch_n(bool forward, const FastArgs<T>* batch, int ntrans, int nz, int maxNcol,
+                   hipStream_t stream) {
+  using S = PlaneShape<T, N>;
+  const size_t lds = S::lds(maxNcol);
+  auto kernel = forward ? k_plane_forward_batch<T, N> : k_plane_backward_batch<T, N>;
+  prepare_lds(kernel, lds);
+  prof::KernelSpan span(stream, forward ? "k_plane_forward_batch" : "k_plane_backward_batch",
+                        true);
+  hipExtLaunchKernelGGL(kernel, dim3(nz * ntrans), dim3(S::NT), lds, stream, span.start(),
+                        span.stop(), 0, batch, nz);
+  check_launch();
+}
+
+#define SPFFT_PLANE_SWITCH(N, CALL) \
+  switch (N) {                      \
+    case 32: CALL(32); break;       \
+    case 64: CALL(64); break;       \
+    case 128: CALL(128); break;     \
+    default: throw GPUFFTError();   \
+  }
+
+template <typename T>
+bool plane_path_supported(int n, int ncol, bool r2c) {
+  if (r2c || !(n == 32 || n == 64 || n == 128)) return false;
+  size_t lds = 0;
+#define SPFFT_PLANE_LDS(N) lds = PlaneShape<T, N>::lds(ncol)
+  SPFFT_PLANE_SWITCH(n, SPFFT_PLANE_LDS)
+#undef SPFFT_PLANE_LDS
+  return lds <= kLdsMax;
+}
+
+template <typename T>
+int plane_pitch(int n, int ncol) {
+  int p = 0;
+#define SPFFT_PLANE_PITCH(N) p = PlaneShape<T, N>::pitch(ncol)
+  SPFFT_PLANE_SWITCH(n, SPFFT_PLANE_PITCH)
+#undef SPFFT_PLANE_PITCH
+  return p;
+}
+
+template <typename T>
+void launch_plane_xy_batch(bool forward, int n, const FastArgs<T>* batch, int ntrans, int nz,
+                           int maxNcol, hipStream_t stream) {
+#define SPFFT_CALL(N) plane_batch_n<T, N>(forward, batch, ntrans, nz, maxNcol, stream)
+  SPFFT_PLANE_SWITCH(n, SPFFT_CALL)
+#undef SPFFT_CALL
+}
+
 #define SPFFT_INSTANTIATE_FAST(T)                                                        \
+  template bool plane_path_supported<T>(int, int, bool);                                 \
+  template int plane_pitch<T>(int, int);                                                 \
+  template void launch_plane_xy_batch<T>(bool, int, const FastArgs<T>*, int, int, int,   \
+                                         hipStream_t);                                   \
   template int fast_xy_pitch_a<T>(int, int);                                             \
   template void launch_fast_z_batch<T>(bool, bool, int, const FastArgs<T>*, const int2*, \
                                        int, hipStream_t);                                \

@@ -54,6 +54,8 @@ struct FastArgs {
   const int* resSticks = nullptr;     // sticks grouped by colX mod R
   const int* resBegin = nullptr;      // R + 1 offsets into resSticks
   const int* resYM = nullptr;         // per resSticks entry: y << 16 | (x / R)
+  const int* tpos = nullptr;          // whole-plane kernels: per stick, y * pitchT + column
+  int pitchT = 0;                     // whole-plane kernels: row pitch of the plane tile
   const Cx<T>* twX = nullptr;
   const Cx<T>* twY = nullptr;
   const Cx<T>* twZ = nullptr;
@@ -94,6 +96,17 @@ void launch_fast_z_batch(bool forward, bool r2c, int nz, const FastArgs<T>* batc
 template <typename T>
 void launch_fast_xy_batch(bool forward, bool r2c, int n, const FastArgs<T>* batch, int ntrans,
                           int nz, int maxNcol, hipStream_t stream);
+
+// Whole-plane kernels for batches (one workgroup per plane; no class split,
+// so the plane's input is read once): power-of-two lengths up to 128, C2C,
+// when the plane tile for `ncol` columns fits in LDS.
+template <typename T>
+bool plane_path_supported(int n, int ncol, bool r2c);
+template <typename T>
+int plane_pitch(int n, int ncol);
+template <typename T>
+void launch_plane_xy_batch(bool forward, int n, const FastArgs<T>* batch, int ntrans, int nz,
+                           int maxNcol, hipStream_t stream);
 
 template <typename T>
 void launch_fast_z_backward(int nz, const FastArgs<T>& a, hipStream_t stream);
