@@ -203,12 +203,9 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
     xCol_ = DeviceBuffer<int>::from_host(xCol);
     resSticks_ = DeviceBuffer<int>::from_host(resSticks);
     resBegin_ = DeviceBuffer<int>::from_host(resBegin);
-    // tile position of every stick for the whole-plane batch kernels
+    // tile pitch for the whole-plane batch kernels (0: not applicable)
     if (plane_path_supported<T>(map_->dimX, numCols_, map_->type == SPFFT_TRANS_R2C)) {
       pitchT_ = plane_pitch<T>(map_->dimX, numCols_);
-      std::vector<int> tpos(map_->num_sticks());
-      for (int s = 0; s < map_->num_sticks(); ++s) tpos[s] = t.stickY[s] * pitchT_ + t.stickCol[s];
-      tpos_ = DeviceBuffer<int>::from_host(tpos);
     }
   }
 
@@ -292,7 +289,6 @@ FastArgs<T> ExecutionGPU<T>::fast_args() const {
   a.resSticks = resSticks_.data();
   a.resBegin = resBegin_.data();
   a.xCol = xCol_.data();
-  a.tpos = tpos_.data();
   a.pitchT = pitchT_;
   a.resYM = resYM_.data();
   a.twX = twX_.data();
@@ -472,7 +468,7 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
   // whole-plane kernels are a candidate when every transform has its tile
   // positions and the widest tile fits
   bool planeOk = plane_kernels_forced() != 0 && plane_path_supported<T>(m0.dimX, maxNcol, r2c);
-  for (size_t t = 0; t < n && planeOk; ++t) planeOk = ex[t]->tpos_.data() != nullptr;
+  for (size_t t = 0; t < n && planeOk; ++t) planeOk = ex[t]->pitchT_ > 0;
   bool plane = false;
   auto launch_xy_variant = [&](bool usePlane, const FastArgs<T>* dArgs, hipStream_t s) {
     if (usePlane) {

@@ -78,8 +78,7 @@ private:
   DeviceBuffer<int> stickAt_;
   DeviceBuffer<int2> zEntries_;
   DeviceBuffer<int> resSticks_, resBegin_, xCol_, resYM_;
-  // whole-plane batch kernels (power-of-two lengths up to 128, C2C)
-  DeviceBuffer<int> tpos_;
+  // whole-plane batch kernels (power-of-two lengths up to 128, C2C): tile pitch
   int pitchT_ = 0;
 };
 

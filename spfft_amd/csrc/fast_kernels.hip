@@ -622,8 +622,11 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
 // T[y][column] (columns of the sparse set plus a zero column) lives in LDS,
 // the y transforms run in column mode on it and the x transforms on rows, so
 // every input element is read from memory once (the class-split kernels read
-// a plane R times). x lines have 4 elements per lane and exchange through a
-// per-wave scratch region after the tile; y lines have 8.
+// a plane R times). Backward: each y line gathers its sticks straight into
+// registers through the stick index table and is written to the tile after
+// its transform; forward: each y line's results go straight to the sticks.
+// x lines have 4 elements per lane and exchange through a per-wave scratch
+// region after the tile; y lines have 8.
 
 template <typename T, int N>
 struct PlaneShape {
@@ -655,25 +658,30 @@ __device__ __forceinline__ void plane_backward_body(const FastArgs<T>& a, int z)
   Cx<T>* twl = xs + S::XROWS * LB;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   if (threadIdx.x < N) twl[threadIdx.x] = a.twX[threadIdx.x];
-  for (int t = threadIdx.x; t < S::ROWS * PU; t += NT) tile[t] = Cx<T>{T(0), T(0)};
   // column of each x this lane handles in the x pass (ncol = zero column)
   int uu[XL::E];
 #pragma unroll
   for (int e = 0; e < XL::E; ++e) uu[e] = a.xCol[lane % XL::P + XL::P * e];
   __syncthreads();
   const Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
-  for (int st = threadIdx.x; st < a.nst; st += NT) tile[a.tpos[st]] = zrow[st];
-  __syncthreads();
-  // y: columns in column mode; columns past ncol are zero and stay zero
+  // y: every column up to and including the zero column ncol, gathered
+  // straight into registers through the stick index table (class-interleaved
+  // for R = 4, M = N / 4: the 8 indices of a lane are two 16-byte loads),
+  // transformed in column mode and written to the tile
   {
+    constexpr int M = N / 4;
+    static_assert(fast_classes(N) == 4 && YL::E == 8 && YL::P * 2 == M, "index layout");
     const int q = lane % YL::P;
-    for (int u0 = wave * YL::PER_WAVE; u0 < ncol; u0 += NW * YL::PER_WAVE) {
+    for (int u0 = wave * YL::PER_WAVE; u0 <= ncol; u0 += NW * YL::PER_WAVE) {
       const int u = u0 + lane / YL::P;
-      Cx<T>* col = tile + u;
+      const int uc = u < ncol ? u : ncol;
+      const int4* rel = reinterpret_cast<const int4*>(a.stickAt) + static_cast<size_t>(uc) * M;
+      const int4 i0 = at32(rel, q), i1 = at32(rel, q + YL::P);
+      const int idx[8] = {i0.x, i1.x, i0.y, i1.y, i0.z, i1.z, i0.w, i1.w};
       Cx<T> v[YL::E];
 #pragma unroll
-      for (int e = 0; e < YL::E; ++e) v[e] = col[(q + YL::P * e) * PU];
-      wave_sync();
+      for (int e = 0; e < YL::E; ++e) v[e] = at32(zrow, idx[e]);
+      Cx<T>* col = tile + u;
       wave_fft<T, N, true, 1, YL::E>(v, col, PU, q, twl);
 #pragma unroll
       for (int e = 0; e < YL::E; ++e) col[(q + YL::P * e) * PU] = v[e];
@@ -735,9 +743,18 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
   __syncthreads();
   // y: the sparse set's columns in column mode
   {
+    // results go straight to the sticks of the column (stick index table as
+    // in the backward kernel; absent positions point at nst and are skipped)
+    constexpr int M = N / 4;
+    static_assert(fast_classes(N) == 4 && YL::E == 8 && YL::P * 2 == M, "index layout");
+    Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
     const int q = lane % YL::P;
     for (int u0 = wave * YL::PER_WAVE; u0 < ncol; u0 += NW * YL::PER_WAVE) {
       const int u = u0 + lane / YL::P;
+      const int uc = u < ncol ? u : ncol;
+      const int4* rel = reinterpret_cast<const int4*>(a.stickAt) + static_cast<size_t>(uc) * M;
+      const int4 i0 = at32(rel, q), i1 = at32(rel, q + YL::P);
+      const int idx[8] = {i0.x, i1.x, i0.y, i1.y, i0.z, i1.z, i0.w, i1.w};
       Cx<T>* col = tile + u;
       Cx<T> v[YL::E];
 #pragma unroll
@@ -745,12 +762,11 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
       wave_sync();
       wave_fft<T, N, false, 1, YL::E>(v, col, PU, q, twl);
 #pragma unroll
-      for (int e = 0; e < YL::E; ++e) col[(q + YL::P * e) * PU] = v[e];
+      for (int e = 0; e < YL::E; ++e) {
+        if (idx[e] < a.nst) zrow[idx[e]] = v[e];
+      }
     }
   }
-  __syncthreads();
-  Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
-  for (int st = threadIdx.x; st < a.nst; st += NT) zrow[st] = tile[a.tpos[st]];
 }
 
 // Arguments read from a device table hold generic pointers, so the compiler
@@ -780,7 +796,6 @@ __device__ __forceinline__ FastArgs<T> on_global(FastArgs<T> a, const void* tabl
   a.resSticks = rebase(g, a.resSticks);
   a.resBegin = rebase(g, a.resBegin);
   a.resYM = rebase(g, a.resYM);
-  a.tpos = rebase(g, a.tpos);
   a.twX = rebase(g, a.twX);
   a.twY = rebase(g, a.twY);
   a.twZ = rebase(g, a.twZ);

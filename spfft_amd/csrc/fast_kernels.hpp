@@ -54,7 +54,6 @@ struct FastArgs {
   const int* resSticks = nullptr;     // sticks grouped by colX mod R
   const int* resBegin = nullptr;      // R + 1 offsets into resSticks
   const int* resYM = nullptr;         // per resSticks entry: y << 16 | (x / R)
-  const int* tpos = nullptr;          // whole-plane kernels: per stick, y * pitchT + column
   int pitchT = 0;                     // whole-plane kernels: row pitch of the plane tile
   const Cx<T>* twX = nullptr;
   const Cx<T>* twY = nullptr;
