@@ -181,6 +181,39 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
       ent[e] = make_int2(t.entryValue[e], pos);
     }
     zEntries_ = DeviceBuffer<int2>::from_host(ent);
+    // Sticks whose values form one run in user order: storage z0, z0 + 1, ...
+    // (cyclically, e.g. centred z ascending) at consecutive value indices.
+    // When every stick is such a run the z kernels address values directly.
+    {
+      const int nz = map_->dimZ;
+      std::vector<int4> runs(map_->num_sticks());
+      bool all = map_->num_sticks() > 0;
+      for (int st = 0; st < map_->num_sticks() && all; ++st) {
+        const int b = t.stickBegin[st], e = t.stickBegin[st + 1];
+        const int len = e - b;
+        // storage z of the entries ascend (sorted by slot); one run has at
+        // most one gap, and starts after it
+        int start = 0;
+        for (int k = b + 1; k < e; ++k) {
+          const int za = t.entrySlot[k - 1] % nz, zb = t.entrySlot[k] % nz;
+          if (zb == za) { all = false; break; }  // repeated triplet
+          if (zb != za + 1) {
+            if (start != 0) { all = false; break; }
+            start = k - b;
+          }
+        }
+        if (!all) break;
+        const int z0 = t.entrySlot[b + start] % nz;
+        const int v0 = t.entryValue[b + start];
+        for (int k = 0; k < len && all; ++k) {
+          const int kk = b + (start + k) % len;
+          const int z = t.entrySlot[kk] % nz;
+          if ((z - z0 + nz) % nz != k || t.entryValue[kk] != v0 + k) all = false;
+        }
+        runs[st] = make_int4(v0, z0, len, 0);
+      }
+      if (all) zInfo_ = DeviceBuffer<int4>::from_host(runs);
+    }
     const int classes = fast_classes(map_->dimX);
     std::vector<int> resBegin(classes + 1, 0), resSticks;
     resSticks.reserve(map_->num_sticks());
@@ -289,6 +322,7 @@ FastArgs<T> ExecutionGPU<T>::fast_args() const {
   a.resSticks = resSticks_.data();
   a.resBegin = resBegin_.data();
   a.xCol = xCol_.data();
+  a.zInfo = zInfo_.data();
   a.pitchT = pitchT_;
   a.resYM = resYM_.data();
   a.twX = twX_.data();

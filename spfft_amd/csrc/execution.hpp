@@ -77,6 +77,7 @@ private:
   bool fast_ = false;
   DeviceBuffer<int> stickAt_;
   DeviceBuffer<int2> zEntries_;
+  DeviceBuffer<int4> zInfo_;  // stick runs, when every stick is one run in user order
   DeviceBuffer<int> resSticks_, resBegin_, xCol_, resYM_;
   // whole-plane batch kernels (power-of-two lengths up to 128, C2C): tile pitch
   int pitchT_ = 0;

@@ -181,6 +181,80 @@ struct ZShape {
   static size_t lds() { return (static_cast<size_t>(TW) + NZ) * sizeof(Cx<T>); }
 };
 
+// Hermitian fill of the (0,0) stick in the tile (reference
+// symmetry_host.hpp:72-89): a non-zero lower entry wins, an upper entry fills
+// a zero lower one, the self-paired entry is conjugated.
+template <typename T, int NZ, int NT>
+__device__ __forceinline__ void fill_zero_zero_stick(Cx<T>* tile) {
+  for (int k = 1 + threadIdx.x; 2 * k <= NZ; k += NT) {
+    Cx<T>& lo = tile[phys<T>(k)];
+    Cx<T>& hi = tile[phys<T>(NZ - k)];
+    if (2 * k == NZ) {
+      if (cnonzero(lo)) lo = cconj(lo);
+    } else if (cnonzero(lo)) {
+      hi = cconj(lo);
+    } else if (cnonzero(hi)) {
+      lo = cconj(hi);
+    }
+  }
+}
+
+// Sticks whose values are one run in user order (zInfo[s] = {first value,
+// first storage z, length}: value of storage z = first + ((z - z0) mod NZ)
+// for the len positions of the run): every lane loads its z values straight
+// into registers, with no entry table, zero fill or scatter.
+template <typename T, int NZ, bool RC>
+__device__ __forceinline__ void z_backward_runs(const FastArgs<T>& a, int bid) {
+  using Z = ZShape<T, NZ>;
+  using L = typename Z::L;
+  constexpr int S = Z::S, LZ = Z::LZ, NT = Z::NT;
+  Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
+  Cx<T>* twl = tile + Z::TW;
+  const int s0 = bid * S;
+  const int ns = min(S, a.nst - s0);
+  const int lane = threadIdx.x % kWave;
+  const int sl = (threadIdx.x / kWave) * L::PER_WAVE + lane / L::P;
+  const int q = lane % L::P;
+  const bool live = s0 + sl < a.nst;
+  const int4 run = a.zInfo[live ? s0 + sl : a.nst - 1];
+  const Cx<T>* in = reinterpret_cast<const Cx<T>*>(a.in);
+  Cx<T> v[L::E];
+#pragma unroll
+  for (int e = 0; e < L::E; ++e) {
+    int pos = q + L::P * e - run.y;
+    pos += pos < 0 ? NZ : 0;
+    const bool present = live && pos < run.z;
+    const Cx<T> x = in[present ? run.x + pos : run.x];
+    v[e] = present ? x : Cx<T>{T(0), T(0)};
+  }
+  if (threadIdx.x < NZ) twl[threadIdx.x] = a.twZ[threadIdx.x];
+  Cx<T>* line = tile + sl * LZ;
+  if (RC && s0 == 0 && a.zeroZeroStick == 0) {
+#pragma unroll
+    for (int e = 0; e < L::E; ++e) line[phys<T>(q + L::P * e)] = v[e];
+    __syncthreads();
+    fill_zero_zero_stick<T, NZ, NT>(tile);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < L::E; ++e) v[e] = line[phys<T>(q + L::P * e)];
+    wave_sync();
+  } else {
+    __syncthreads();  // twiddles
+  }
+  // the xy kernels read element nst of every row as zero
+  if (bid == 0 && threadIdx.x < NZ) {
+    a.zbuf[static_cast<size_t>(threadIdx.x) * a.pitch + a.nst] = Cx<T>{T(0), T(0)};
+  }
+  wave_fft<T, NZ, true, 1, L::E>(v, line, 1, q, twl);
+#pragma unroll
+  for (int e = 0; e < L::E; ++e) line[phys<T>(q + L::P * e)] = v[e];
+  __syncthreads();
+  for (int t = threadIdx.x; t < S * NZ; t += NT) {
+    const int s = t % S, zz = t / S;
+    if (s < ns) a.zbuf[static_cast<size_t>(zz) * a.pitch + s0 + s] = tile[s * LZ + phys<T>(zz)];
+  }
+}
+
 template <typename T, int NZ, bool RC>
 __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
   using Z = ZShape<T, NZ>;
@@ -190,6 +264,10 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
   Cx<T>* twl = tile + Z::TW;
   const int s0 = bid * S;
   const int ns = min(S, a.nst - s0);
+  if (a.zInfo) {
+    z_backward_runs<T, NZ, RC>(a, bid);
+    return;
+  }
   const int e0 = a.stickBegin[s0], e1 = a.stickBegin[s0 + ns];
   // Entries carry their tile position (a repeated triplet's earlier copies
   // point at the spare slot after the tile, so the last one wins). All of a
@@ -223,20 +301,7 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
   }
   __syncthreads();
   if (RC && s0 == 0 && a.zeroZeroStick == 0) {
-    // Hermitian fill of the (0,0) stick (reference symmetry_host.hpp:72-89):
-    // a non-zero lower entry wins, an upper entry fills a zero lower one, the
-    // self-paired entry is conjugated.
-    for (int k = 1 + threadIdx.x; 2 * k <= NZ; k += NT) {
-      Cx<T>& lo = tile[phys<T>(k)];
-      Cx<T>& hi = tile[phys<T>(NZ - k)];
-      if (2 * k == NZ) {
-        if (cnonzero(lo)) lo = cconj(lo);
-      } else if (cnonzero(lo)) {
-        hi = cconj(lo);
-      } else if (cnonzero(hi)) {
-        lo = cconj(hi);
-      }
-    }
+    fill_zero_zero_stick<T, NZ, NT>(tile);
     __syncthreads();
   }
   // the xy kernels read element nst of every row as zero
@@ -276,6 +341,30 @@ __device__ __forceinline__ void z_forward_body(const FastArgs<T>& a, int bid) {
                                         : Cx<T>{T(0), T(0)};
   }
   if (threadIdx.x < NZ) twl[threadIdx.x] = a.twZ[threadIdx.x];
+  if (a.zInfo) {
+    // sticks stored as runs: each lane writes its outputs straight to the
+    // run (see z_backward_runs)
+    const int lane = threadIdx.x % kWave;
+    const int sl = (threadIdx.x / kWave) * L::PER_WAVE + lane / L::P;
+    const int q = lane % L::P;
+    const bool live = s0 + sl < a.nst;
+    const int4 run = a.zInfo[live ? s0 + sl : a.nst - 1];
+    __syncthreads();
+    Cx<T>* line = tile + sl * LZ;
+    Cx<T> v[L::E];
+#pragma unroll
+    for (int e = 0; e < L::E; ++e) v[e] = line[phys<T>(q + L::P * e)];
+    wave_sync();
+    wave_fft<T, NZ, false, 1, L::E>(v, line, 1, q, twl);
+    Cx<T>* out = reinterpret_cast<Cx<T>*>(a.out);
+#pragma unroll
+    for (int e = 0; e < L::E; ++e) {
+      int pos = q + L::P * e - run.y;
+      pos += pos < 0 ? NZ : 0;
+      if (live && pos < run.z) out[run.x + pos] = a.useScale ? cscale(v[e], a.scale) : v[e];
+    }
+    return;
+  }
   // the gather's entries are loaded while the sticks arrive
   const int e0 = a.stickBegin[s0], e1 = a.stickBegin[s0 + ns];
   int ev[K], ep[K];
@@ -639,7 +728,9 @@ struct PlaneShape {
   static constexpr int XROWS = NW * XL::PER_WAVE;
   static_assert(N <= NT, "one twiddle per thread");
   static_assert(N % XROWS == 0, "whole x iterations");
-  static int pitch(int ncol) { return round_up(round_up(ncol + 1, YL::PER_WAVE) - 4, 16) + 4; }
+  // odd pitch: the 16 lanes of an LDS cycle in column mode (16 rows of one
+  // column) land on distinct bank groups
+  static int pitch(int ncol) { return round_up(ncol + 1, YL::PER_WAVE) | 1; }
   static size_t lds(int ncol) {
     return (static_cast<size_t>(ROWS) * pitch(ncol) + static_cast<size_t>(XROWS) * LB + N) *
            sizeof(Cx<T>);
@@ -792,6 +883,7 @@ __device__ __forceinline__ FastArgs<T> on_global(FastArgs<T> a, const void* tabl
   a.colBegin = rebase(g, a.colBegin);
   a.stickAt = rebase(g, a.stickAt);
   a.zEntries = rebase(g, a.zEntries);
+  a.zInfo = rebase(g, a.zInfo);
   a.xCol = rebase(g, a.xCol);
   a.resSticks = rebase(g, a.resSticks);
   a.resBegin = rebase(g, a.resBegin);

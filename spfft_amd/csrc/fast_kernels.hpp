@@ -50,6 +50,8 @@ struct FastArgs {
                                       // or nst (the zero element); row ncol is all nst
   const int2* zEntries = nullptr;     // per value (sorted by slot): {value index,
                                       //  LDS tile position | repeated << 31}
+  const int4* zInfo = nullptr;        // per stick {first value, first storage z, length} when
+                                      // every stick's values are one run in user order
   const int* xCol = nullptr;          // [NX]: column id of x, or ncol (a zero column)
   const int* resSticks = nullptr;     // sticks grouped by colX mod R
   const int* resBegin = nullptr;      // R + 1 offsets into resSticks
