@@ -719,10 +719,12 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
 
 template <typename T, int N>
 struct PlaneShape {
-  static constexpr int NT = N >= 128 ? 1024 : (N >= 64 ? 512 : 256);
+  // powers of two: x lines of 4 elements, y lines of 8; 96: 12 and 12
+  static constexpr bool F3 = N % 3 == 0;
+  static constexpr int NT = F3 ? 768 : (N >= 128 ? 1024 : (N >= 64 ? 512 : 256));
   static constexpr int NW = NT / kWave;
-  using XL = Line<N, 4>;
-  using YL = Line<N, 8>;
+  using XL = Line<N, F3 ? 12 : 4>;
+  using YL = Line<N, F3 ? 12 : 8>;
   static constexpr int ROWS = padded_len<T>(N);  // the column exchange reaches padded rows
   static constexpr int LB = padded_len<T>(N);    // x line in the scratch region
   static constexpr int XROWS = NW * XL::PER_WAVE;
@@ -736,6 +738,25 @@ struct PlaneShape {
            sizeof(Cx<T>);
   }
 };
+
+// Stick indices of the y line of column u held by lane q (y = q + P e): the
+// table is class-interleaved for R = 4, M = N / 4 (index of y = j + M l at
+// (u M + j) R + l), so lane q reads M / P 16-byte groups g, each holding
+// e = g + (M / P) l for l < 4.
+template <int N, typename YL>
+__device__ __forceinline__ void plane_stick_indices(const int* table, int u, int q, int* idx) {
+  constexpr int M = N / 4, G = M / YL::P;
+  static_assert(fast_classes(N) == 4 && M % YL::P == 0 && YL::E == 4 * G, "index layout");
+  const int4* rel = reinterpret_cast<const int4*>(table) + static_cast<size_t>(u) * M;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int4 x = at32(rel, q + YL::P * g);
+    idx[g] = x.x;
+    idx[g + G] = x.y;
+    idx[g + 2 * G] = x.z;
+    idx[g + 3 * G] = x.w;
+  }
+}
 
 template <typename T, int N>
 __device__ __forceinline__ void plane_backward_body(const FastArgs<T>& a, int z) {
@@ -760,15 +781,12 @@ __device__ __forceinline__ void plane_backward_body(const FastArgs<T>& a, int z)
   // for R = 4, M = N / 4: the 8 indices of a lane are two 16-byte loads),
   // transformed in column mode and written to the tile
   {
-    constexpr int M = N / 4;
-    static_assert(fast_classes(N) == 4 && YL::E == 8 && YL::P * 2 == M, "index layout");
     const int q = lane % YL::P;
     for (int u0 = wave * YL::PER_WAVE; u0 <= ncol; u0 += NW * YL::PER_WAVE) {
       const int u = u0 + lane / YL::P;
       const int uc = u < ncol ? u : ncol;
-      const int4* rel = reinterpret_cast<const int4*>(a.stickAt) + static_cast<size_t>(uc) * M;
-      const int4 i0 = at32(rel, q), i1 = at32(rel, q + YL::P);
-      const int idx[8] = {i0.x, i1.x, i0.y, i1.y, i0.z, i1.z, i0.w, i1.w};
+      int idx[YL::E];
+      plane_stick_indices<N, YL>(a.stickAt, uc, q, idx);
       Cx<T> v[YL::E];
 #pragma unroll
       for (int e = 0; e < YL::E; ++e) v[e] = at32(zrow, idx[e]);
@@ -836,16 +854,13 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
   {
     // results go straight to the sticks of the column (stick index table as
     // in the backward kernel; absent positions point at nst and are skipped)
-    constexpr int M = N / 4;
-    static_assert(fast_classes(N) == 4 && YL::E == 8 && YL::P * 2 == M, "index layout");
     Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
     const int q = lane % YL::P;
     for (int u0 = wave * YL::PER_WAVE; u0 < ncol; u0 += NW * YL::PER_WAVE) {
       const int u = u0 + lane / YL::P;
       const int uc = u < ncol ? u : ncol;
-      const int4* rel = reinterpret_cast<const int4*>(a.stickAt) + static_cast<size_t>(uc) * M;
-      const int4 i0 = at32(rel, q), i1 = at32(rel, q + YL::P);
-      const int idx[8] = {i0.x, i1.x, i0.y, i1.y, i0.z, i1.z, i0.w, i1.w};
+      int idx[YL::E];
+      plane_stick_indices<N, YL>(a.stickAt, uc, q, idx);
       Cx<T>* col = tile + u;
       Cx<T> v[YL::E];
 #pragma unroll
@@ -1189,13 +1204,14 @@ void plane_batch_n(bool forward, const FastArgs<T>* batch, int ntrans, int nz, i
   switch (N) {                      \
     case 32: CALL(32); break;       \
     case 64: CALL(64); break;       \
+    case 96: CALL(96); break;       \
     case 128: CALL(128); break;     \
     default: throw GPUFFTError();   \
   }
 
 template <typename T>
 bool plane_path_supported(int n, int ncol, bool r2c) {
-  if (r2c || !(n == 32 || n == 64 || n == 128)) return false;
+  if (r2c || !(n == 32 || n == 64 || n == 96 || n == 128)) return false;
   size_t lds = 0;
 #define SPFFT_PLANE_LDS(N) lds = PlaneShape<T, N>::lds(ncol)
   SPFFT_PLANE_SWITCH(n, SPFFT_PLANE_LDS)
