@@ -774,23 +774,30 @@ __device__ __forceinline__ void plane_backward_body(const FastArgs<T>& a, int z)
   int uu[XL::E];
 #pragma unroll
   for (int e = 0; e < XL::E; ++e) uu[e] = a.xCol[lane % XL::P + XL::P * e];
-  __syncthreads();
   const Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
   // y: every column up to and including the zero column ncol, gathered
-  // straight into registers through the stick index table (class-interleaved
-  // for R = 4, M = N / 4: the 8 indices of a lane are two 16-byte loads),
-  // transformed in column mode and written to the tile
+  // straight into registers through the stick index table, transformed in
+  // column mode and written to the tile. The next column group's gathers
+  // are issued before the current group's transform (software pipelining).
   {
     const int q = lane % YL::P;
-    for (int u0 = wave * YL::PER_WAVE; u0 <= ncol; u0 += NW * YL::PER_WAVE) {
-      const int u = u0 + lane / YL::P;
-      const int uc = u < ncol ? u : ncol;
+    auto gather = [&](int g0, Cx<T>* dst) {
+      const int u = g0 + lane / YL::P;
       int idx[YL::E];
-      plane_stick_indices<N, YL>(a.stickAt, uc, q, idx);
+      plane_stick_indices<N, YL>(a.stickAt, u < ncol ? u : ncol, q, idx);
+#pragma unroll
+      for (int e = 0; e < YL::E; ++e) dst[e] = at32(zrow, idx[e]);
+    };
+    int u0 = wave * YL::PER_WAVE;
+    Cx<T> nv[YL::E];
+    if (u0 <= ncol) gather(u0, nv);
+    __syncthreads();  // twiddles
+    for (; u0 <= ncol; u0 += NW * YL::PER_WAVE) {
       Cx<T> v[YL::E];
 #pragma unroll
-      for (int e = 0; e < YL::E; ++e) v[e] = at32(zrow, idx[e]);
-      Cx<T>* col = tile + u;
+      for (int e = 0; e < YL::E; ++e) v[e] = nv[e];
+      if (u0 + NW * YL::PER_WAVE <= ncol) gather(u0 + NW * YL::PER_WAVE, nv);
+      Cx<T>* col = tile + u0 + lane / YL::P;
       wave_fft<T, N, true, 1, YL::E>(v, col, PU, q, twl);
 #pragma unroll
       for (int e = 0; e < YL::E; ++e) col[(q + YL::P * e) * PU] = v[e];
@@ -829,18 +836,28 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
   int uu[XL::E];
 #pragma unroll
   for (int e = 0; e < XL::E; ++e) uu[e] = a.xCol[lane % XL::P + XL::P * e];
-  __syncthreads();
-  // x: every row, read once; the sparse set's columns go into the tile
+  // x: every row, read once; the sparse set's columns go into the tile. The
+  // next rows' loads are issued before the current rows' transform.
   {
     const int q = lane % XL::P;
     const int lr = wave * XL::PER_WAVE + lane / XL::P;
     Cx<T>* line = xs + lr * LB;
     const Cx<T>* in = reinterpret_cast<const Cx<T>*>(a.in) + static_cast<size_t>(z) * N * N;
+    Cx<T> nv[XL::E];
+#pragma unroll
+    for (int e = 0; e < XL::E; ++e) nv[e] = in[static_cast<size_t>(lr) * N + q + XL::P * e];
+    __syncthreads();  // twiddles
     for (int y0 = 0; y0 < N; y0 += S::XROWS) {
       const int y = y0 + lr;
       Cx<T> v[XL::E];
 #pragma unroll
-      for (int e = 0; e < XL::E; ++e) v[e] = in[static_cast<size_t>(y) * N + q + XL::P * e];
+      for (int e = 0; e < XL::E; ++e) v[e] = nv[e];
+      if (y0 + S::XROWS < N) {
+#pragma unroll
+        for (int e = 0; e < XL::E; ++e) {
+          nv[e] = in[static_cast<size_t>(y + S::XROWS) * N + q + XL::P * e];
+        }
+      }
       wave_fft<T, N, false, 1, XL::E>(v, line, 1, q, twl);
 #pragma unroll
       for (int e = 0; e < XL::E; ++e) {
