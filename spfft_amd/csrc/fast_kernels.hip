@@ -1,4 +1,4 @@
-// Fixed-length kernels of the C2C sparse 3D FFT for gfx950 (see
+// Fixed-length kernels of the sparse 3D FFT (C2C and R2C) for gfx950 (see
 // fast_kernels.hpp for the decomposition).
 //
 // FFT engine (wave level): a line of N points is owned by P = N / E lanes of
@@ -6,7 +6,7 @@
 // Stockham pass applies radix-R butterflies to lane-local elements (R divides
 // E, so every butterfly's inputs sit in one lane); all passes but the last
 // exchange results through the line's own LDS region, whose index map
-// i -> i + i / 8 (FP64; i / 16 for FP32) keeps the stride-R writes free of
+// i -> i + i / 8 keeps most of the stride-R writes free of
 // bank conflicts. The last pass leaves the line in natural order in the
 // lanes' registers. Exchanges are wave-local (no workgroup barrier), so the
 // waves of a workgroup run their FFTs independently.
@@ -36,7 +36,8 @@ constexpr int kWave = 64;
 // ---- line shapes and radix schedules (compile time)
 
 // Elements per lane: 8 for powers of two, 24 for lengths with a factor 3.
-// The z kernels use half of that (more lanes per line, more waves in flight).
+// The FP64 z kernels use half of that (more lanes per line, more waves in
+// flight).
 constexpr int elems_per_lane(int n) { return n % 3 == 0 ? 24 : 8; }
 constexpr int z_elems_per_lane(int n) { return n % 3 == 0 ? 12 : 4; }
 
@@ -57,10 +58,12 @@ constexpr int wave_radix(int m, int e) {
                                       : -1;
 }
 
-// Padded index inside an exchange region: one spare element every 128 bytes.
+// Padded index inside an exchange region: one spare element every 8
+// elements (128 bytes in FP64, 64 bytes in FP32; the denser FP32 padding
+// measured slightly faster than one per 128 bytes).
 template <typename T>
 constexpr int pad_shift() {
-  return sizeof(T) == 8 ? 3 : 4;
+  return 3;
 }
 template <typename T>
 __device__ __forceinline__ int phys(int i) {
@@ -170,7 +173,10 @@ __device__ __forceinline__ void plane_class(int b, int& z, int& c) {
 
 template <typename T, int NZ>
 struct ZShape {
-  using L = Line<NZ, z_elems_per_lane(NZ)>;
+  // FP64 and lengths with a factor 3: narrow lines (more waves in flight);
+  // FP32 powers of two: full lines, so a workgroup owns 16 sticks and its
+  // [z][stick] row segments are whole 128-byte lines
+  using L = Line<NZ, (sizeof(T) == 4 && NZ % 3 != 0) ? elems_per_lane(NZ) : z_elems_per_lane(NZ)>;
   static constexpr int NT = 256;
   static constexpr int S = (NT / kWave) * L::PER_WAVE;  // sticks per workgroup
   static constexpr int LZ = padded_len<T>(NZ);
