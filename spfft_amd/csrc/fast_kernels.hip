@@ -432,11 +432,9 @@ struct XYShape {
   // the rows have room for the spare columns of a partial last wave
   static constexpr int LT = make_odd(padded_len<T>(round_up(M, Line<N>::PER_WAVE)));
   // Row pitch of A: whole waves of columns including the zero column ncol,
-  // and == 4 (mod 16) so that the lanes of a 16-lane LDS phase hit distinct
-  // bank groups.
-  static int pitch_a(int ncol) {
-    return round_up(round_up(ncol + 1, YL::PER_WAVE) - 4, 16) + 4;
-  }
+  // and odd, so that the 8 lanes of one column in an FP64 LDS cycle (rows
+  // q = 0..7 of the y line) hit distinct bank groups.
+  static int pitch_a(int ncol) { return round_up(ncol + 1, YL::PER_WAVE) | 1; }
   // elements of the A/B region, followed by the twiddle table (N) and, for
   // R2C, the filled x = 0 column (N)
   static size_t tile_backward(int ncol) {
