@@ -50,7 +50,8 @@ struct Line {
 };
 
 constexpr int wave_radix(int m, int e) {
-  return (m % 8 == 0 && e % 8 == 0)   ? 8
+  return (m % 16 == 0 && e % 16 == 0) ? 16
+         : (m % 8 == 0 && e % 8 == 0) ? 8
          : (m % 4 == 0 && e % 4 == 0) ? 4
          : (m % 2 == 0 && e % 2 == 0) ? 2
          : (m % 3 == 0 && e % 3 == 0) ? 3
@@ -69,6 +70,20 @@ template <typename T>
 __device__ __forceinline__ int phys(int i) {
   return i + (i >> pad_shift<T>());
 }
+// Exchange slot of element i: phys(i) (SW = 0), or, for lines of 8 lanes
+// holding 2^SW elements each, one spare slot per 2^SW elements:
+// slot(i) = i + (i >> SW). The 8 lanes of such a line then reach 8 distinct
+// slots mod 8 in every exchange write (i = 2^SW q + r: (2^SW + 1) q + r) and
+// read (i = q + 8 e), and slot offsets stay affine in r and e (immediate
+// offsets of one base address).
+template <typename T, int SW>
+__device__ __forceinline__ int slot(int i) {
+  if constexpr (SW == 0) {
+    return phys<T>(i);
+  } else {
+    return i + (i >> SW);
+  }
+}
 template <typename T>
 constexpr int padded_len(int n) {
   return n + ((n - 1) >> pad_shift<T>());
@@ -84,19 +99,37 @@ __device__ __forceinline__ X at32(const X* base, int i) {
   return *reinterpret_cast<const X*>(reinterpret_cast<const char*>(base) + off);
 }
 
+// Workgroup barrier for LDS data: waits for this wave's LDS operations only.
+// __syncthreads() releases at workgroup scope for all memory, which waits for
+// every global load and store in flight (loads issued ahead of the barrier
+// would land before it). No kernel here passes data between waves through
+// global memory.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+struct NoHook {
+  __device__ void operator()() const {}
+};
+
 // One Stockham pass (and, recursively, the remaining ones) of the wave-level
-// FFT. `line` is this line's exchange region: element i at line[phys(i) * es]
-// (rows: es = 1; columns of a tile: es = row length). The twiddle table holds
-// W_{N * TWS}. NS = product of the radices already applied.
-template <typename T, int N, bool BWD, int TWS, int E = elems_per_lane(N), int NS = 1>
+// FFT. `line` is this line's exchange region: element i at
+// line[slot<SW>(i) * es] (rows: es = 1; columns of a tile: es = row length).
+// The twiddle table holds W_{N * TWS}. NS = product of the radices already
+// applied. `hook` runs once, after the first pass has written its outputs to
+// LDS (the line's registers are free then): callers issue loads there.
+template <typename T, int N, bool BWD, int TWS, int E = elems_per_lane(N), int NS = 1,
+          int SW = 0, typename Hook = NoHook>
 __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
-                                         const Cx<T>* __restrict__ tw) {
+                                         const Cx<T>* __restrict__ tw, Hook hook = Hook()) {
   constexpr int PL = N / E;
   constexpr int R = wave_radix(N / NS, E);
   static_assert(R > 0, "no radix divides the lane's element count");
@@ -123,19 +156,20 @@ __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int i = o + r * NS;
-        line[phys<T>(i) * es] = u[r];
+        line[slot<T, SW>(i) * es] = u[r];
       }
     }
   }
   if constexpr (!LAST) {
+    hook();
     wave_sync();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = q + PL * e;
-      v[e] = line[phys<T>(i) * es];
+      v[e] = line[slot<T, SW>(i) * es];
     }
     wave_sync();
-    wave_fft<T, N, BWD, TWS, E, NS * R>(v, line, es, q, tw);
+    wave_fft<T, N, BWD, TWS, E, NS * R, SW>(v, line, es, q, tw);
   }
 }
 
@@ -896,6 +930,80 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
   }
 }
 
+// ------------------------------------------------- full-tile plane kernels
+//
+// One workgroup per plane with the whole plane in LDS: tile row y holds the
+// plane's columns at their compact positions u < ncol, and, with row pitch
+// N + 1, room for the x line of row y, which runs in place (no exchange
+// scratch). Lines have 8 lanes and E = N / 8 elements per lane, so every y
+// and x line is one Stockham exchange and all lines of a phase run at once
+// (one line per lane group). Exchange slots have one spare per E elements
+// (slot<SW>, ROWS slots per line), and the two lines sharing an LDS cycle
+// (lanes 0-7 and 8-15 of a wave) are 8 columns or 8 rows apart, so they use
+// opposite halves of the banks. Element ROWS of each row is zero: absent x
+// positions read it.
+//
+// Each workgroup takes planes blockIdx.x, blockIdx.x + gridDim.x, ... The
+// FP32 launches are persistent (as many workgroups as fit on the GPU at once;
+// at N = 128 the tile allows one per CU), and the loads of a workgroup's next
+// plane (backward: its y lines' gathers, forward: its rows) are issued during
+// the current plane's second phase. The FP64 launches have one workgroup per
+// plane. Barriers are LDS-only, so loads in flight cross them.
+
+template <typename T, int N>
+struct FullShape {
+  static constexpr int E = N / 8;
+  static constexpr int SW = N == 128 ? 4 : 3;
+  using L = Line<N, E>;
+  static_assert(L::P == 8 && (1 << SW) == E, "lines of 8 lanes, power-of-two elements");
+  static constexpr int ROWS = N + N / E;        // exchange slots of a line
+  static constexpr int PU = (ROWS + 1) | 1;     // odd row pitch, zero element at ROWS
+  static constexpr int NW = N / L::PER_WAVE;  // one line per row (column)
+  static constexpr int NT = NW * kWave;
+  // persistent grid (FP32), or one plane per workgroup (FP64, where staggered
+  // workgroup starts measured faster than the prefetching plane loop)
+  static constexpr bool PERSISTENT = sizeof(T) == 4;
+  // tile, twiddle table, x -> compact column map of the current plane
+  static constexpr size_t lds() {
+    return (static_cast<size_t>(ROWS) * PU + N) * sizeof(Cx<T>) + N * sizeof(int);
+  }
+};
+
+// The tile offset of column `line`, opaque to the compiler: column addresses
+// reach past the 64 KiB immediate offset of LDS instructions, and without
+// this the loop-invariant part of every one of them would be hoisted out of
+// the plane loop into a register of its own.
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+// Row (column) of lane group g of wave w: the groups sharing an LDS cycle,
+// g = 2k and 2k + 1, are 8 apart.
+__device__ __forceinline__ int full_line(int w, int g) {
+  return 16 * (w >> 1) + 4 * (w & 1) + (g >> 1) + 8 * (g & 1);
+}
+
+// Values of y line `line` of plane z (a column of the sparse set), gathered
+// from the [z][stick] buffer through its stick indices.
+template <typename T, typename L>
+__device__ __forceinline__ void full_gather(const FastArgs<T>& a, int z, const int* idx,
+                                            Cx<T>* v) {
+  const Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
+#pragma unroll
+  for (int e = 0; e < L::E; ++e) v[e] = at32(zrow, idx[e]);
+}
+
+// Values of x line `line` (a row) of plane z of the input grid.
+template <typename T, int N, typename L>
+__device__ __forceinline__ void full_rows(const FastArgs<T>& a, int z, int line, int q,
+                                          Cx<T>* v) {
+  const Cx<T>* in =
+      reinterpret_cast<const Cx<T>*>(a.in) + (static_cast<size_t>(z) * N + line) * N;
+#pragma unroll
+  for (int e = 0; e < L::E; ++e) v[e] = in[q + L::P * e];
+}
+
 // Arguments read from a device table hold generic pointers, so the compiler
 // would access them with flat instructions (which share the LDS counter and
 // serialise against LDS traffic). Rebasing each pointer on a kernel argument
@@ -987,6 +1095,200 @@ __global__ void __launch_bounds__((PlaneShape<T, N>::NT))
     k_plane_forward_batch(const FastArgs<T>* __restrict__ batch, int nz) {
   const int t = blockIdx.x / nz;
   plane_forward_body<T, N>(on_global(batch[t], batch), blockIdx.x - t * nz);
+}
+// Entry t of an argument table, read through the constant address space:
+// the table is not written while a kernel runs, and loads inside a loop with
+// global stores would otherwise be vector loads (whose wait also waits for
+// every load issued before them).
+template <typename T>
+__device__ __forceinline__ FastArgs<T> table_args(const FastArgs<T>* batch, int t) {
+  static_assert(sizeof(FastArgs<T>) % 4 == 0, "whole dwords");
+  using ConstWords = const __attribute__((address_space(4))) unsigned*;
+  // t is uniform; saying so keeps the loads scalar where the compiler cannot
+  // prove it (inside the FFT hooks)
+  const ConstWords src = (ConstWords)(batch + __builtin_amdgcn_readfirstlane(t));
+  struct Words {
+    unsigned w[sizeof(FastArgs<T>) / 4];
+  } words;
+#pragma unroll
+  for (int i = 0; i < static_cast<int>(sizeof(FastArgs<T>) / 4); ++i) words.w[i] = src[i];
+  return on_global(__builtin_bit_cast(FastArgs<T>, words), batch);
+}
+
+// A uniform value, loaded into scalar registers here: a table field used
+// only under divergent control flow would otherwise be loaded there, with a
+// vector load whose wait also waits for every load issued before it.
+template <typename X>
+__device__ __forceinline__ X pin(X x) {
+  asm volatile("" : "+s"(x));
+  return x;
+}
+// The same for a pointer into global memory: rebased on a kernel argument so
+// that the accesses stay global (not flat) ones.
+template <typename P>
+__device__ __forceinline__ P* pin(const void* anchor, P* p) {
+  return rebase(static_cast<const char*>(anchor), pin(p));
+}
+
+// Full-tile kernels: `total` = planes of the batch (transforms x nz), the
+// grid at most that. Every transform of a batch has the same N, so the
+// twiddle table is loaded once.
+template <typename T, int N>
+__global__ void __launch_bounds__((FullShape<T, N>::NT))
+    k_full_backward_batch(const FastArgs<T>* __restrict__ batch, int nz, int total) {
+  using S = FullShape<T, N>;
+  using L = typename S::L;
+  constexpr int E = L::E, P = L::P, PU = S::PU, ROWS = S::ROWS;
+  Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
+  Cx<T>* twl = tile + ROWS * PU;
+  int* xmap = reinterpret_cast<int*>(twl + N);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int q = lane % P;
+  const int line = full_line(wave, lane / P);
+  const int line0 = full_line(wave, 0);  // the wave's lowest line
+  const int xi = threadIdx.x & (N - 1);  // x of this thread's map entry
+  const int G = gridDim.x;
+  // In flight at the top of the loop: plane p's y line values (v) and map
+  // entry (xc), plane p + G's stick indices (idx). Every lane loads: lines
+  // past ncol read the zero column of the index table (all entries nst, the
+  // zero element), so no load sits under a divergent branch.
+  int p = blockIdx.x;
+  int t = p / nz;
+  FastArgs<T> a = table_args(batch, t);
+  if (threadIdx.x < N) {
+    twl[threadIdx.x] = a.twX[threadIdx.x];
+    tile[threadIdx.x * PU + ROWS] = Cx<T>{T(0), T(0)};
+  }
+  Cx<T> v[E];
+  int idx[E];
+  plane_stick_indices<N, L>(a.stickAt, min(line, a.ncol), q, idx);
+  full_gather<T, L>(a, p - t * nz, idx, v);
+  int xc = a.xCol[xi];
+  int tn = p + G < total ? (p + G) / nz : t;
+  FastArgs<T> an = table_args(batch, tn);
+  plane_stick_indices<N, L>(an.stickAt, min(line, an.ncol), q, idx);
+  for (;;) {
+    const int z = p - t * nz;
+    const int ncol = a.ncol;
+    lds_barrier();  // tile and map free (first plane: twiddles, zero elements)
+    // y: column `line`, from the registers gathered earlier (waves whose
+    // lines are all past ncol skip it; the others transform their idle lines
+    // too, into tile columns nobody reads)
+    if (line0 < ncol) {
+      Cx<T>* col = tile + opaque(line);
+      wave_fft<T, N, true, 1, E, 1, S::SW>(v, col, PU, q, twl);
+#pragma unroll
+      for (int e = 0; e < E; ++e) col[(q + P * e) * PU] = v[e];
+    }
+    if (threadIdx.x < N) xmap[threadIdx.x] = xc < ncol ? xc : ROWS;
+    lds_barrier();
+    // x: row `line`, in place. Once its first pass is in LDS, plane p + G's
+    // gathers (indices already in registers) and map entry, and plane
+    // p + 2G's indices go out.
+    Cx<T>* row = tile + line * PU;
+    Cx<T> x[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = row[xmap[q + P * e]];
+    Cx<T>* out = reinterpret_cast<Cx<T>*>(a.out) + (static_cast<size_t>(z) * N + line) * N;
+    const int pn = p + G;
+    wave_fft<T, N, true, 1, E, 1, S::SW>(x, row, 1, q, twl, [&] {
+      if (pn < total) {
+        full_gather<T, L>(an, pn - tn * nz, idx, v);
+        xc = pin(batch, an.xCol)[xi];
+        const int pnn = pn + G;
+        const FastArgs<T> ann = table_args(batch, pnn < total ? pnn / nz : tn);
+        plane_stick_indices<N, L>(pin(batch, ann.stickAt), min(line, pin(ann.ncol)), q, idx);
+      }
+    });
+#pragma unroll
+    for (int e = 0; e < E; ++e) out[q + P * e] = x[e];
+    if (pn >= total) break;
+    p = pn;
+    t = tn;
+    a = an;
+    tn = pn + G < total ? (pn + G) / nz : t;
+    an = table_args(batch, tn);
+  }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__((FullShape<T, N>::NT))
+    k_full_forward_batch(const FastArgs<T>* __restrict__ batch, int nz, int total) {
+  using S = FullShape<T, N>;
+  using L = typename S::L;
+  constexpr int E = L::E, P = L::P, PU = S::PU;
+  Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
+  Cx<T>* twl = tile + S::ROWS * PU;
+  int* xmap = reinterpret_cast<int*>(twl + N);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int q = lane % P;
+  const int line = full_line(wave, lane / P);
+  const int line0 = full_line(wave, 0);
+  const int xi = threadIdx.x & (N - 1);
+  const int G = gridDim.x;
+  // In flight at the top of the loop: plane p's rows (v).
+  int p = blockIdx.x;
+  int t = p / nz;
+  FastArgs<T> a = table_args(batch, t);
+  Cx<T> v[E];
+  full_rows<T, N, L>(a, p - t * nz, line, q, v);
+  if (threadIdx.x < N) {
+    twl[threadIdx.x] = a.twX[threadIdx.x];
+    xmap[threadIdx.x] = a.xCol[threadIdx.x];
+  }
+  for (;;) {
+    const int z = p - t * nz;
+    const int ncol = a.ncol;
+    lds_barrier();  // tile and map free (first plane: twiddles)
+    // x: row `line`, in place; the sparse set's columns stay in the row
+    Cx<T>* row = tile + line * PU;
+    wave_fft<T, N, false, 1, E, 1, S::SW>(v, row, 1, q, twl);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int u = xmap[q + P * e];
+      if (u < ncol) row[u] = v[e];
+    }
+    lds_barrier();
+    // y: column `line`, results straight to its sticks. Waves whose lines
+    // are all past ncol skip the transform; the others transform their idle
+    // lines too and write them, like absent positions, to element nst (the
+    // backward z pass rewrites it with zero). Once the first pass is in LDS,
+    // the next plane's rows and map entry and this line's stick indices go
+    // out.
+    const int pn = p + G;
+    const int tn = pn < total ? pn / nz : t;
+    const FastArgs<T> an = table_args(batch, tn);
+    int xc = 0;
+    int idx[E];
+    auto prefetch = [&] {
+      if (pn < total) {
+        xc = pin(batch, an.xCol)[xi];
+        full_rows<T, N, L>(an, pn - tn * nz, line, q, v);
+      }
+    };
+    if (line0 < ncol) {
+      const int* stickAt = pin(batch, a.stickAt);
+      Cx<T>* col = tile + opaque(line);
+      Cx<T> w[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) w[e] = col[(q + P * e) * PU];
+      wave_sync();
+      wave_fft<T, N, false, 1, E, 1, S::SW>(w, col, PU, q, twl, [&] {
+        prefetch();
+        plane_stick_indices<N, L>(stickAt, min(line, ncol), q, idx);
+      });
+      Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
+#pragma unroll
+      for (int e = 0; e < E; ++e) zrow[idx[e]] = w[e];
+    } else {
+      prefetch();
+    }
+    if (pn >= total) break;
+    if (threadIdx.x < N) xmap[threadIdx.x] = xc;
+    p = pn;
+    t = tn;
+    a = an;
+  }
 }
 
 // Raise the dynamic LDS limit of a kernel once per process (the attribute
@@ -1207,9 +1509,40 @@ void launch_fast_xy_batch(bool forward, bool r2c, int n, const FastArgs<T>* batc
   }
 }
 
+// The full-tile kernels serve N = 64 and 128 when the N x (N + 1) tile fits.
+template <typename T, int N>
+constexpr bool full_tile_fits() {
+  if constexpr (N == 64 || N == 128) {
+    return FullShape<T, N>::lds() <= kLdsMax;
+  } else {
+    return false;
+  }
+}
+
 template <typename T, int N>
 void plane_batch_n(bool forward, const FastArgs<T>* batch, int ntrans, int nz, int maxNcol,
                    hipStream_t stream) {
+  if constexpr (full_tile_fits<T, N>()) {
+    using F = FullShape<T, N>;
+    auto kernel = forward ? k_full_forward_batch<T, N> : k_full_backward_batch<T, N>;
+    prepare_lds(kernel, F::lds());
+    // persistent grid: the workgroups that are resident at once
+    const int total = nz * ntrans;
+    int grid = total;
+    if (F::PERSISTENT) {
+      int device = 0, cus = 0, perCu = 0;
+      check_hip(hipGetDevice(&device));
+      check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+      check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, F::NT, F::lds()));
+      grid = std::max(1, std::min(total, std::max(1, perCu) * cus));
+    }
+    prof::KernelSpan span(stream, forward ? "k_full_forward_batch" : "k_full_backward_batch",
+                          true);
+    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(F::NT), F::lds(), stream, span.start(),
+                          span.stop(), 0, batch, nz, total);
+    check_launch();
+    return;
+  }
   using S = PlaneShape<T, N>;
   const size_t lds = S::lds(maxNcol);
   auto kernel = forward ? k_plane_forward_batch<T, N> : k_plane_backward_batch<T, N>;
@@ -1234,7 +1567,7 @@ template <typename T>
 bool plane_path_supported(int n, int ncol, bool r2c) {
   if (r2c || !(n == 32 || n == 64 || n == 96 || n == 128)) return false;
   size_t lds = 0;
-#define SPFFT_PLANE_LDS(N) lds = PlaneShape<T, N>::lds(ncol)
+#define SPFFT_PLANE_LDS(N) lds = full_tile_fits<T, N>() ? 0 : PlaneShape<T, N>::lds(ncol)
   SPFFT_PLANE_SWITCH(n, SPFFT_PLANE_LDS)
 #undef SPFFT_PLANE_LDS
   return lds <= kLdsMax;

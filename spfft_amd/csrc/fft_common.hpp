@@ -110,6 +110,40 @@ struct Butterfly<T, 8, BWD> {
   }
 };
 
+// 16 = 2 x 8: X[k] = E[k] + W16^k O[k], X[k + 8] = E[k] - W16^k O[k] with E, O
+// the length-8 DFTs of the even and odd inputs.
+template <typename T, bool BWD>
+struct Butterfly<T, 16, BWD> {
+  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>* w, int n) {
+    Cx<T> e[8], o[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      e[k] = v[2 * k];
+      o[k] = v[2 * k + 1];
+    }
+    Butterfly<T, 8, BWD>::run(e, w, n);
+    Butterfly<T, 8, BWD>::run(o, w, n);
+    // cos and sin of 2 pi k / 16, k = 0..7
+    constexpr T c[8] = {T(1), T(0.92387953251128675613), T(0.70710678118654752440),
+                        T(0.38268343236508977173), T(0), T(-0.38268343236508977173),
+                        T(-0.70710678118654752440), T(-0.92387953251128675613)};
+    constexpr T s[8] = {T(0), T(0.38268343236508977173), T(0.70710678118654752440),
+                        T(0.92387953251128675613), T(1), T(0.92387953251128675613),
+                        T(0.70710678118654752440), T(0.38268343236508977173)};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      Cx<T> t = o[k];
+      if (k == 4) {
+        t = rot<BWD>(t);
+      } else if (k != 0) {
+        t = cmul(t, Cx<T>{c[k], BWD ? s[k] : -s[k]});
+      }
+      v[k] = cadd(e[k], t);
+      v[k + 8] = csub(e[k], t);
+    }
+  }
+};
+
 template <typename T, bool BWD>
 struct Butterfly<T, 3, BWD> {
   __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
