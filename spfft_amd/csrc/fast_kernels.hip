@@ -110,6 +110,22 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Streaming store (no cache allocation on the way to HBM): for batch outputs
+// that no later kernel of the chain reads while they could still be cached.
+template <typename T>
+__device__ __forceinline__ void store_stream(Cx<T>* p, Cx<T> v) {
+  __builtin_nontemporal_store(v.re, &p->re);
+  __builtin_nontemporal_store(v.im, &p->im);
+}
+template <bool STREAM, typename T>
+__device__ __forceinline__ void store(Cx<T>* p, Cx<T> v) {
+  if constexpr (STREAM) {
+    store_stream(p, v);
+  } else {
+    *p = v;
+  }
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -243,7 +259,7 @@ __device__ __forceinline__ void fill_zero_zero_stick(Cx<T>* tile) {
 // first storage z, length}: value of storage z = first + ((z - z0) mod NZ)
 // for the len positions of the run): every lane loads its z values straight
 // into registers, with no entry table, zero fill or scatter.
-template <typename T, int NZ, bool RC>
+template <typename T, int NZ, bool RC, bool STREAM>
 __device__ __forceinline__ void z_backward_runs(const FastArgs<T>& a, int bid) {
   using Z = ZShape<T, NZ>;
   using L = typename Z::L;
@@ -291,11 +307,13 @@ __device__ __forceinline__ void z_backward_runs(const FastArgs<T>& a, int bid) {
   __syncthreads();
   for (int t = threadIdx.x; t < S * NZ; t += NT) {
     const int s = t % S, zz = t / S;
-    if (s < ns) a.zbuf[static_cast<size_t>(zz) * a.pitch + s0 + s] = tile[s * LZ + phys<T>(zz)];
+    if (s < ns) store<STREAM>(&a.zbuf[static_cast<size_t>(zz) * a.pitch + s0 + s], tile[s * LZ + phys<T>(zz)]);
   }
 }
 
-template <typename T, int NZ, bool RC>
+// STREAM: streaming stores of the [z][stick] rows (batches, whose
+// intermediates exceed the caches anyway)
+template <typename T, int NZ, bool RC, bool STREAM>
 __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
   using Z = ZShape<T, NZ>;
   using L = typename Z::L;
@@ -305,7 +323,7 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
   const int s0 = bid * S;
   const int ns = min(S, a.nst - s0);
   if (a.zInfo) {
-    z_backward_runs<T, NZ, RC>(a, bid);
+    z_backward_runs<T, NZ, RC, STREAM>(a, bid);
     return;
   }
   const int e0 = a.stickBegin[s0], e1 = a.stickBegin[s0 + ns];
@@ -362,7 +380,7 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
   __syncthreads();
   for (int t = threadIdx.x; t < S * NZ; t += NT) {
     const int s = t % S, zz = t / S;
-    if (s < ns) a.zbuf[static_cast<size_t>(zz) * a.pitch + s0 + s] = tile[s * LZ + phys<T>(zz)];
+    if (s < ns) store<STREAM>(&a.zbuf[static_cast<size_t>(zz) * a.pitch + s0 + s], tile[s * LZ + phys<T>(zz)]);
   }
 }
 
@@ -1045,7 +1063,7 @@ __device__ __forceinline__ FastArgs<T> on_global(FastArgs<T> a, const void* tabl
 
 template <typename T, int NZ, bool RC>
 __global__ void __launch_bounds__((ZShape<T, NZ>::NT)) k_z_backward_fast(FastArgs<T> a) {
-  z_backward_body<T, NZ, RC>(a, blockIdx.x);
+  z_backward_body<T, NZ, RC, false>(a, blockIdx.x);
 }
 template <typename T, int NZ>
 __global__ void __launch_bounds__((ZShape<T, NZ>::NT)) k_z_forward_fast(FastArgs<T> a) {
@@ -1063,7 +1081,7 @@ template <typename T, int NZ, bool RC>
 __global__ void __launch_bounds__((ZShape<T, NZ>::NT))
     k_z_backward_batch(const FastArgs<T>* __restrict__ batch, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
-  z_backward_body<T, NZ, RC>(on_global(batch[m.x], batch), m.y);
+  z_backward_body<T, NZ, RC, true>(on_global(batch[m.x], batch), m.y);
 }
 template <typename T, int NZ>
 __global__ void __launch_bounds__((ZShape<T, NZ>::NT))
@@ -1201,7 +1219,7 @@ __global__ void __launch_bounds__((FullShape<T, N>::NT))
       }
     });
 #pragma unroll
-    for (int e = 0; e < E; ++e) out[q + P * e] = x[e];
+    for (int e = 0; e < E; ++e) store_stream(&out[q + P * e], x[e]);
     if (pn >= total) break;
     p = pn;
     t = tn;
