@@ -50,13 +50,19 @@ struct Line {
 };
 
 constexpr int wave_radix(int m, int e) {
-  return (m % 16 == 0 && e % 16 == 0) ? 16
-         : (m % 8 == 0 && e % 8 == 0) ? 8
+  return (m % 8 == 0 && e % 8 == 0)   ? 8
          : (m % 4 == 0 && e % 4 == 0) ? 4
          : (m % 2 == 0 && e % 2 == 0) ? 2
          : (m % 3 == 0 && e % 3 == 0) ? 3
          : (m % 5 == 0 && e % 5 == 0) ? 5
                                       : -1;
+}
+// Lines of the full-tile kernels take the whole lane's elements in their
+// first pass (radix 16 or 24 in registers) so that a line needs one exchange.
+constexpr int full_radix(int m, int e) {
+  return (m % 24 == 0 && e % 24 == 0)   ? 24
+         : (m % 16 == 0 && e % 16 == 0) ? 16
+                                        : wave_radix(m, e);
 }
 
 // Padded index inside an exchange region: one spare element every 8
@@ -70,18 +76,18 @@ template <typename T>
 __device__ __forceinline__ int phys(int i) {
   return i + (i >> pad_shift<T>());
 }
-// Exchange slot of element i: phys(i) (SW = 0), or, for lines of 8 lanes
-// holding 2^SW elements each, one spare slot per 2^SW elements:
-// slot(i) = i + (i >> SW). The 8 lanes of such a line then reach 8 distinct
-// slots mod 8 in every exchange write (i = 2^SW q + r: (2^SW + 1) q + r) and
-// read (i = q + 8 e), and slot offsets stay affine in r and e (immediate
-// offsets of one base address).
-template <typename T, int SW>
+// Exchange slot of element i: phys(i) (PE = 0), or, for the full-tile lines
+// of P lanes holding PE elements each, one spare slot per PE elements:
+// slot(i) = i + i / PE. The P lanes of such a line then reach P distinct
+// slots mod P in every exchange write (i = PE q + r: (PE + 1) q + r) and read
+// (i = q + P e), and slot offsets stay affine in r and e (immediate offsets
+// of one base address).
+template <typename T, int PE>
 __device__ __forceinline__ int slot(int i) {
-  if constexpr (SW == 0) {
+  if constexpr (PE == 0) {
     return phys<T>(i);
   } else {
-    return i + (i >> SW);
+    return i + static_cast<int>(static_cast<unsigned>(i) / PE);
   }
 }
 template <typename T>
@@ -138,16 +144,16 @@ struct NoHook {
 
 // One Stockham pass (and, recursively, the remaining ones) of the wave-level
 // FFT. `line` is this line's exchange region: element i at
-// line[slot<SW>(i) * es] (rows: es = 1; columns of a tile: es = row length).
+// line[slot<PE>(i) * es] (rows: es = 1; columns of a tile: es = row length).
 // The twiddle table holds W_{N * TWS}. NS = product of the radices already
 // applied. `hook` runs once, after the first pass has written its outputs to
 // LDS (the line's registers are free then): callers issue loads there.
 template <typename T, int N, bool BWD, int TWS, int E = elems_per_lane(N), int NS = 1,
-          int SW = 0, typename Hook = NoHook>
+          int PE = 0, typename Hook = NoHook>
 __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
                                          const Cx<T>* __restrict__ tw, Hook hook = Hook()) {
   constexpr int PL = N / E;
-  constexpr int R = wave_radix(N / NS, E);
+  constexpr int R = PE > 0 ? full_radix(N / NS, E) : wave_radix(N / NS, E);
   static_assert(R > 0, "no radix divides the lane's element count");
   constexpr int EB = E / R;
   constexpr bool LAST = NS * R == N;
@@ -172,7 +178,7 @@ __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int i = o + r * NS;
-        line[slot<T, SW>(i) * es] = u[r];
+        line[slot<T, PE>(i) * es] = u[r];
       }
     }
   }
@@ -182,10 +188,10 @@ __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = q + PL * e;
-      v[e] = line[slot<T, SW>(i) * es];
+      v[e] = line[slot<T, PE>(i) * es];
     }
     wave_sync();
-    wave_fft<T, N, BWD, TWS, E, NS * R, SW>(v, line, es, q, tw);
+    wave_fft<T, N, BWD, TWS, E, NS * R, PE>(v, line, es, q, tw);
   }
 }
 
@@ -952,14 +958,14 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
 //
 // One workgroup per plane with the whole plane in LDS: tile row y holds the
 // plane's columns at their compact positions u < ncol, and, with row pitch
-// N + 1, room for the x line of row y, which runs in place (no exchange
-// scratch). Lines have 8 lanes and E = N / 8 elements per lane, so every y
-// and x line is one Stockham exchange and all lines of a phase run at once
-// (one line per lane group). Exchange slots have one spare per E elements
-// (slot<SW>, ROWS slots per line), and the two lines sharing an LDS cycle
-// (lanes 0-7 and 8-15 of a wave) are 8 columns or 8 rows apart, so they use
-// opposite halves of the banks. Element ROWS of each row is zero: absent x
-// positions read it.
+// PU > ROWS, room for the x line of row y, which runs in place (no exchange
+// scratch). Lines have P lanes and E = N / P elements per lane (64: 8 x 8,
+// 96: 4 x 24, 128: 8 x 16), so every y and x line is one Stockham exchange
+// after an in-register first pass, and all lines of a phase run at once (one
+// line per lane group). Exchange slots have one spare per E elements
+// (slot<E>, ROWS slots per line), and the lines sharing an LDS cycle are 8
+// (or 4) columns or rows apart, so they use disjoint banks. Element ROWS of
+// each row is zero: absent x positions read it.
 //
 // Each workgroup takes planes blockIdx.x, blockIdx.x + gridDim.x, ... The
 // FP32 launches are persistent (as many workgroups as fit on the GPU at once;
@@ -970,13 +976,15 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
 
 template <typename T, int N>
 struct FullShape {
-  static constexpr int E = N / 8;
-  static constexpr int SW = N == 128 ? 4 : 3;
+  // lines of 8 lanes (64: 8 x 8, 128: 16 x 8) or 4 lanes (96: 24 x 4)
+  static constexpr int E = N == 96 ? 24 : N / 8;
   using L = Line<N, E>;
-  static_assert(L::P == 8 && (1 << SW) == E, "lines of 8 lanes, power-of-two elements");
-  static constexpr int ROWS = N + N / E;        // exchange slots of a line
+  static constexpr int P = L::P;
+  static_assert(P == 8 || P == 4, "lines of 4 or 8 lanes");
+  static constexpr int PE = E;                  // one spare slot per E elements
+  static constexpr int ROWS = N + (N - 1) / E;  // exchange slots of a line
   static constexpr int PU = (ROWS + 1) | 1;     // odd row pitch, zero element at ROWS
-  static constexpr int NW = N / L::PER_WAVE;  // one line per row (column)
+  static constexpr int NW = N / L::PER_WAVE;    // one line per row (column)
   static constexpr int NT = NW * kWave;
   // persistent grid (FP32), or one plane per workgroup (FP64, where staggered
   // workgroup starts measured faster than the prefetching plane loop)
@@ -996,10 +1004,16 @@ __device__ __forceinline__ int opaque(int x) {
   return x;
 }
 
-// Row (column) of lane group g of wave w: the groups sharing an LDS cycle,
-// g = 2k and 2k + 1, are 8 apart.
+// Row (column) of lane group g of wave w. An LDS cycle serves 16 lanes: with
+// 8 lanes per line its two lines, g = 2k and 2k + 1, are 8 apart; with 4
+// lanes per line its four lines, g = 4k .. 4k + 3, are 4 apart.
+template <int P>
 __device__ __forceinline__ int full_line(int w, int g) {
-  return 16 * (w >> 1) + 4 * (w & 1) + (g >> 1) + 8 * (g & 1);
+  if constexpr (P == 8) {
+    return 16 * (w >> 1) + 4 * (w & 1) + (g >> 1) + 8 * (g & 1);
+  } else {
+    return 16 * w + 4 * (g & 3) + (g >> 2);
+  }
 }
 
 // Values of y line `line` of plane z (a column of the sparse set), gathered
@@ -1162,9 +1176,9 @@ __global__ void __launch_bounds__((FullShape<T, N>::NT))
   int* xmap = reinterpret_cast<int*>(twl + N);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int q = lane % P;
-  const int line = full_line(wave, lane / P);
-  const int line0 = full_line(wave, 0);  // the wave's lowest line
-  const int xi = threadIdx.x & (N - 1);  // x of this thread's map entry
+  const int line = full_line<P>(wave, lane / P);
+  const int line0 = full_line<P>(wave, 0);  // the wave's lowest line
+  const int xi = static_cast<int>(threadIdx.x % N);  // x of this thread's map entry
   const int G = gridDim.x;
   // In flight at the top of the loop: plane p's y line values (v) and map
   // entry (xc), plane p + G's stick indices (idx). Every lane loads: lines
@@ -1194,7 +1208,7 @@ __global__ void __launch_bounds__((FullShape<T, N>::NT))
     // too, into tile columns nobody reads)
     if (line0 < ncol) {
       Cx<T>* col = tile + opaque(line);
-      wave_fft<T, N, true, 1, E, 1, S::SW>(v, col, PU, q, twl);
+      wave_fft<T, N, true, 1, E, 1, S::PE>(v, col, PU, q, twl);
 #pragma unroll
       for (int e = 0; e < E; ++e) col[(q + P * e) * PU] = v[e];
     }
@@ -1209,7 +1223,7 @@ __global__ void __launch_bounds__((FullShape<T, N>::NT))
     for (int e = 0; e < E; ++e) x[e] = row[xmap[q + P * e]];
     Cx<T>* out = reinterpret_cast<Cx<T>*>(a.out) + (static_cast<size_t>(z) * N + line) * N;
     const int pn = p + G;
-    wave_fft<T, N, true, 1, E, 1, S::SW>(x, row, 1, q, twl, [&] {
+    wave_fft<T, N, true, 1, E, 1, S::PE>(x, row, 1, q, twl, [&] {
       if (pn < total) {
         full_gather<T, L>(an, pn - tn * nz, idx, v);
         xc = pin(batch, an.xCol)[xi];
@@ -1240,9 +1254,9 @@ __global__ void __launch_bounds__((FullShape<T, N>::NT))
   int* xmap = reinterpret_cast<int*>(twl + N);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int q = lane % P;
-  const int line = full_line(wave, lane / P);
-  const int line0 = full_line(wave, 0);
-  const int xi = threadIdx.x & (N - 1);
+  const int line = full_line<P>(wave, lane / P);
+  const int line0 = full_line<P>(wave, 0);
+  const int xi = static_cast<int>(threadIdx.x % N);
   const int G = gridDim.x;
   // In flight at the top of the loop: plane p's rows (v).
   int p = blockIdx.x;
@@ -1260,7 +1274,7 @@ __global__ void __launch_bounds__((FullShape<T, N>::NT))
     lds_barrier();  // tile and map free (first plane: twiddles)
     // x: row `line`, in place; the sparse set's columns stay in the row
     Cx<T>* row = tile + line * PU;
-    wave_fft<T, N, false, 1, E, 1, S::SW>(v, row, 1, q, twl);
+    wave_fft<T, N, false, 1, E, 1, S::PE>(v, row, 1, q, twl);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int u = xmap[q + P * e];
@@ -1291,7 +1305,7 @@ __global__ void __launch_bounds__((FullShape<T, N>::NT))
 #pragma unroll
       for (int e = 0; e < E; ++e) w[e] = col[(q + P * e) * PU];
       wave_sync();
-      wave_fft<T, N, false, 1, E, 1, S::SW>(w, col, PU, q, twl, [&] {
+      wave_fft<T, N, false, 1, E, 1, S::PE>(w, col, PU, q, twl, [&] {
         prefetch();
         plane_stick_indices<N, L>(stickAt, min(line, ncol), q, idx);
       });
@@ -1527,10 +1541,11 @@ void launch_fast_xy_batch(bool forward, bool r2c, int n, const FastArgs<T>* batc
   }
 }
 
-// The full-tile kernels serve N = 64 and 128 when the N x (N + 1) tile fits.
+// The full-tile kernels serve N = 64 and 128 when the tile fits, and 96 in
+// FP32 (FP64 lines of 24 elements do not fit in the register file).
 template <typename T, int N>
 constexpr bool full_tile_fits() {
-  if constexpr (N == 64 || N == 128) {
+  if constexpr (N == 64 || N == 128 || (N == 96 && sizeof(T) == 4)) {
     return FullShape<T, N>::lds() <= kLdsMax;
   } else {
     return false;

@@ -144,6 +144,13 @@ struct Butterfly<T, 16, BWD> {
   }
 };
 
+// 24 = 8 x 3: A[n2] = the length-8 DFT of x[3 n1 + n2], then
+// X[k1 + 8 k2] = sum_n2 W3^(n2 k2) W24^(n2 k1) A[n2][k1].
+template <typename T, bool BWD>
+struct Butterfly<T, 24, BWD> {
+  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>* w, int n);
+};
+
 template <typename T, bool BWD>
 struct Butterfly<T, 3, BWD> {
   __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
@@ -178,6 +185,40 @@ struct Butterfly<T, 5, BWD> {
     v[3] = csub(b2, d2);
   }
 };
+
+template <typename T, bool BWD>
+__device__ __forceinline__ void Butterfly<T, 24, BWD>::run(Cx<T>* v, const Cx<T>* w, int n) {
+  Cx<T> a[3][8];
+#pragma unroll
+  for (int n2 = 0; n2 < 3; ++n2) {
+#pragma unroll
+    for (int n1 = 0; n1 < 8; ++n1) a[n2][n1] = v[3 * n1 + n2];
+    Butterfly<T, 8, BWD>::run(a[n2], w, n);
+  }
+  // cos and sin of 2 pi m / 24, m = 0..14
+  constexpr T c15 = T(0.96592582628906828675), s15 = T(0.25881904510252076235);
+  constexpr T c30 = T(0.86602540378443864676), c45 = T(0.70710678118654752440);
+  constexpr T c[15] = {T(1),  c15,  c30,  c45,  T(0.5),  s15,  T(0),    -s15,
+                       T(-0.5), -c45, -c30, -c15, T(-1),   -c15, -c30};
+  constexpr T s[15] = {T(0), s15, T(0.5), c45, c30, c15, T(1), c15,
+                       c30,  c45, T(0.5), s15, T(0), -s15, T(-0.5)};
+#pragma unroll
+  for (int n2 = 1; n2 < 3; ++n2) {
+#pragma unroll
+    for (int k1 = 1; k1 < 8; ++k1) {
+      const int m = n2 * k1;
+      a[n2][k1] = cmul(a[n2][k1], Cx<T>{c[m], BWD ? s[m] : -s[m]});
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 8; ++k1) {
+    Cx<T> t[3] = {a[0][k1], a[1][k1], a[2][k1]};
+    Butterfly<T, 3, BWD>::run(t, w, n);
+    v[k1] = t[0];
+    v[k1 + 8] = t[1];
+    v[k1 + 16] = t[2];
+  }
+}
 
 }  // namespace dev
 }  // namespace spfft
