@@ -424,6 +424,49 @@ def test_batched_graph_replays(precision, monkeypatch):
             assert orc.rel_l2(f, o.forward(want, True)) <= tol, (rep, k)
 
 
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_batched_whole_plane_many_planes_per_workgroup(precision, monkeypatch):
+    """16 uneven 64^3 transforms (1024 planes, more than the FP32 full-tile
+    kernels' resident grid of one launch, so their workgroups loop over
+    several planes of different transforms and prefetch across them). Two
+    rounds with new inputs, with and without graph replay; every result
+    matches the oracle and the replays match direct launches bit for bit."""
+    monkeypatch.setenv("SPFFT_AMD_PLANE", "1")
+    n = 64
+    dims = (n, n, n)
+    shifts = uniform(700, 16 * 3).reshape(16, 3) * 0.5
+    radii = 12.0 + 9.0 * uniform01(701, 16)
+    sets = [sphere_triplets(float(radii[k]), kshift=tuple(shifts[k])) for k in range(16)]
+    trs = [build(dims, t, False, precision) for t in sets]
+    dt = torch.float64 if precision == "double" else torch.float32
+    rd = rdt(precision)
+    ins = [torch.empty(2 * len(t), dtype=dt, device="cuda") for t in sets]
+    spaces = [torch.empty(2 * n ** 3, dtype=dt, device="cuda") for _ in trs]
+    outs = [torch.empty(2 * len(t), dtype=dt, device="cuda") for t in sets]
+    tol = FP64_TOL if precision == "double" else FP32_TOL
+    for rep in range(2):
+        vals = [complex_values(720 + 20 * rep + k, len(t)) for k, t in enumerate(sets)]
+        for k, v in enumerate(vals):
+            ins[k].copy_(dev(v.view(np.float64).astype(rd)))
+        torch.cuda.synchronize()
+        results = []
+        for graphs in ("1", "0"):
+            monkeypatch.setenv("SPFFT_AMD_GRAPHS", graphs)
+            sp.multi_transform_backward(trs, ins, spaces)
+            sp.multi_transform_forward(trs, spaces, outs, [sp.SPFFT_FULL_SCALING] * len(trs))
+            torch.cuda.synchronize()
+            results.append(([s.cpu().numpy().copy() for s in spaces], [o.cpu().numpy().copy() for o in outs]))
+        for k, t in enumerate(sets):
+            np.testing.assert_array_equal(results[0][0][k], results[1][0][k])
+            np.testing.assert_array_equal(results[0][1][k], results[1][1][k])
+            o = orc.OraclePlan(False, dims, t)
+            want = o.backward(vals[k])
+            got = results[0][0][k].astype(np.float64).view(np.complex128).reshape(n, n, n)
+            assert orc.rel_l2(got, want) <= tol, (rep, k)
+            f = results[0][1][k].astype(np.float64).view(np.complex128)
+            assert orc.rel_l2(f, o.forward(want, True)) <= tol, (rep, k)
+
+
 GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
 
 
