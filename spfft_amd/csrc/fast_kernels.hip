@@ -1574,17 +1574,17 @@ void plane_batch_n(bool forward, const FastArgs<T>* batch, int ntrans, int nz, i
     hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(F::NT), F::lds(), stream, span.start(),
                           span.stop(), 0, batch, nz, total);
     check_launch();
-    return;
+  } else {
+    using S = PlaneShape<T, N>;
+    const size_t lds = S::lds(maxNcol);
+    auto kernel = forward ? k_plane_forward_batch<T, N> : k_plane_backward_batch<T, N>;
+    prepare_lds(kernel, lds);
+    prof::KernelSpan span(stream, forward ? "k_plane_forward_batch" : "k_plane_backward_batch",
+                          true);
+    hipExtLaunchKernelGGL(kernel, dim3(nz * ntrans), dim3(S::NT), lds, stream, span.start(),
+                          span.stop(), 0, batch, nz);
+    check_launch();
   }
-  using S = PlaneShape<T, N>;
-  const size_t lds = S::lds(maxNcol);
-  auto kernel = forward ? k_plane_forward_batch<T, N> : k_plane_backward_batch<T, N>;
-  prepare_lds(kernel, lds);
-  prof::KernelSpan span(stream, forward ? "k_plane_forward_batch" : "k_plane_backward_batch",
-                        true);
-  hipExtLaunchKernelGGL(kernel, dim3(nz * ntrans), dim3(S::NT), lds, stream, span.start(),
-                        span.stop(), 0, batch, nz);
-  check_launch();
 }
 
 #define SPFFT_PLANE_SWITCH(N, CALL) \
