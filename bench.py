@@ -165,6 +165,37 @@ def batched_bench(args, sp, torch, np):
     return out
 
 
+def launch_floor(torch, kernel_us):
+    """Single-launch floor at the size of the dominant kernel's grid: a fill
+    (write only) and a copy of the 128^3 complex FP64 grid (33.5 MB) by
+    torch's own kernels, rotating over 8 buffers, timed with HIP events.
+    `kernel_frac_of_fill` = fill time / dominant kernel time."""
+    n = 2 * 128 ** 3
+    bufs = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(8)]
+    out = {"bytes": n * 8}
+    for name, fn in (("fill", lambda i: bufs[i % 8].fill_(1.0)),
+                     ("copy", lambda i: bufs[i % 8].copy_(bufs[(i + 3) % 8]))):
+        for i in range(20):
+            fn(i)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for i in range(200):
+            fn(i)
+        e1.record()
+        torch.cuda.synchronize()
+        out[f"{name}_us"] = e0.elapsed_time(e1) / 200 * 1e3
+    out["fill_gbs"] = out["bytes"] / out["fill_us"] / 1e3
+    out["kernel_frac_of_fill"] = out["fill_us"] / kernel_us
+    del bufs
+    torch.cuda.empty_cache()
+    log(f"launch floor: fill {out['fill_us']:.2f} us ({out['fill_gbs']:.0f} GB/s), copy "
+        f"{out['copy_us']:.2f} us for 33.5 MB; dominant kernel at "
+        f"{out['kernel_frac_of_fill'] * 100:.0f}% of the fill rate")
+    return out
+
+
 def plain_r2c_bench(args, sp, torch, np):
     """128^3 R2C FP64, half sphere r=40 (x > 0, or x = 0 and y > 0, or
     x = y = 0 and z >= 0): single transforms, backward+forward pairs/s."""
@@ -404,6 +435,7 @@ def main():
         },
         "kernels": kernels,
     }
+    result["launch_floor"] = launch_floor(torch, d["mean_us"])
     result["plain_r2c"] = plain_r2c_bench(args, sp, torch, np)
     if args.batch > 0:
         b = batched_bench(args, sp, torch, np)
