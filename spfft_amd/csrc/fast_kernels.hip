@@ -123,6 +123,13 @@ __device__ __forceinline__ void store_stream(Cx<T>* p, Cx<T> v) {
   __builtin_nontemporal_store(v.re, &p->re);
   __builtin_nontemporal_store(v.im, &p->im);
 }
+// Two consecutive FP32 elements (16 bytes, 16-byte aligned) in one streaming
+// store.
+__device__ __forceinline__ void store_stream_pair(Cx<float>* p, Cx<float> a, Cx<float> b) {
+  using F4 = __attribute__((ext_vector_type(4))) float;
+  const F4 w = {a.re, a.im, b.re, b.im};
+  __builtin_nontemporal_store(w, reinterpret_cast<F4*>(p));
+}
 template <bool STREAM, typename T>
 __device__ __forceinline__ void store(Cx<T>* p, Cx<T> v) {
   if constexpr (STREAM) {
@@ -1232,8 +1239,39 @@ __global__ void __launch_bounds__((FullShape<T, N>::NT))
         plane_stick_indices<N, L>(pin(batch, ann.stickAt), min(line, pin(ann.ncol)), q, idx);
       }
     });
+    if constexpr (sizeof(T) == 4 && P == 8) {
+      // 128-byte row segments per store instruction: the row goes back to
+      // LDS in natural order and each lane stores two consecutive elements
 #pragma unroll
-    for (int e = 0; e < E; ++e) store_stream(&out[q + P * e], x[e]);
+      for (int e = 0; e < E; ++e) row[q + P * e] = x[e];
+      wave_sync();
+#pragma unroll
+      for (int k = 0; k < E / 2; ++k) {
+        const int i = 2 * (q + P * k);
+        store_stream_pair(out + i, row[i], row[i + 1]);
+      }
+    } else if constexpr (sizeof(T) == 4 && P == 4) {
+      // 4-lane lines: the wave's 16 rows (16 w .. 16 w + 15) go back to LDS
+      // in natural order, then 8 lanes per row store two consecutive
+      // elements each: 8 rows x 128 bytes per store instruction
+#pragma unroll
+      for (int e = 0; e < E; ++e) row[q + P * e] = x[e];
+      wave_sync();
+      Cx<T>* plane = reinterpret_cast<Cx<T>*>(a.out) + static_cast<size_t>(z) * N * N;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int r = 16 * wave + lane / 8 + 8 * j;
+        const Cx<T>* src = tile + r * PU;
+#pragma unroll
+        for (int k = 0; k < N / 16; ++k) {
+          const int i = 2 * (lane % 8 + 8 * k);
+          store_stream_pair(plane + static_cast<size_t>(r) * N + i, src[i], src[i + 1]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) store_stream(&out[q + P * e], x[e]);
+    }
     if (pn >= total) break;
     p = pn;
     t = tn;
