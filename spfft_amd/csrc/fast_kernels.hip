@@ -976,14 +976,14 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
 //
 // Each workgroup takes planes blockIdx.x, blockIdx.x + gridDim.x, ... The
 // FP32 launches are persistent (as many workgroups as fit on the GPU at once;
-// at N = 128 the tile allows one per CU), and the loads of a workgroup's next
+// one per CU at 96 and 128), and the loads of a workgroup's next
 // plane (backward: its y lines' gathers, forward: its rows) are issued during
 // the current plane's second phase. The FP64 launches have one workgroup per
 // plane. Barriers are LDS-only, so loads in flight cross them.
 
 template <typename T, int N>
 struct FullShape {
-  // lines of 8 lanes (64: 8 x 8, 128: 16 x 8) or 4 lanes (96: 24 x 4)
+  // lanes x elements per line: 64: 8 x 8, 96: 4 x 24, 128: 8 x 16
   static constexpr int E = N == 96 ? 24 : N / 8;
   using L = Line<N, E>;
   static constexpr int P = L::P;
