@@ -64,12 +64,41 @@ def read_profile(lib):
     return out
 
 
+def host_cpu():
+    """CPU model and physical core count of this host (from /proc/cpuinfo)."""
+    model, cores = "unknown", set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model == "unknown":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None and core is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None and core is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return model, len(cores) or (os.cpu_count() or 1)
+
+
 def cpu_baseline(t, dims, values, seconds):
-    """The C oracle on the host cores: warm once, then pairs until `seconds`."""
-    import numpy as np
+    """The C oracle (restatement of the reference's SPFFT_PU_HOST path) on the
+    host: warm once, then backward+forward pairs until `seconds`. Threads =
+    OMP_NUM_THREADS (the CPU share this job may use), else the physical cores."""
     from oracle import oracle as orc
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    model, physical = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or physical
+    orc.use_fft_library()  # the FFTW3 API as the reference calls it, when the host has one
     plan = orc.OraclePlan(False, dims, t, nthreads=threads)
+    provider = orc.fft_provider()
     space = plan.backward(values)
     plan.forward(space, True)
     pairs = 0
@@ -82,8 +111,10 @@ def cpu_baseline(t, dims, values, seconds):
         if el >= seconds:
             break
     return {"value": pairs / el, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "physical_cores": physical, "fft_provider": provider,
             "sample": f"{pairs} backward+forward pairs of the same 128^3 r=40 C2C FP64 workload "
-                      f"({el:.1f} s), C/OpenMP restatement of the reference host path"}
+                      f"({el:.1f} s) through the C/OpenMP restatement of the reference host "
+                      f"path on {threads} threads, 1D FFTs by {provider}"}
 
 
 def _bench_transforms(sp, torch, np, trs, ins, outs, steps, r2c=False):
@@ -153,16 +184,57 @@ def batched_bench(args, sp, torch, np):
             sec = _bench_transforms(sp, torch, np, trs, ins, outs, steps)
             rb = np.dtype(real).itemsize
             nv = sum(len(t) for t in sets) * bands
-            pair_bytes = 2 * (nv * (2 * rb + 4) + len(trs) * n ** 3 * 2 * rb)
+            # compulsory bytes: values and dense grid per band, the value map
+            # once per distinct index set (bands of a set share it)
+            maps = sum(len(t) for t in sets) * 4
+            pair_bytes = 2 * (nv * 2 * rb + maps + len(trs) * n ** 3 * 2 * rb)
             rate = len(trs) / sec
+            err = batch_error(sp, torch, np, trs, ins, outs, sets, bands, dims, prec)
             rec[prec] = {"band_pairs_per_s": rate, "ms_per_call_pair": sec * 1e3,
-                         "frac_hbm": pair_bytes / sec / 1e9 / HBM_PEAK_GBS}
+                         "frac_hbm": pair_bytes / sec / 1e9 / HBM_PEAK_GBS, **err}
             log(f"batched {name} {prec}: {rate:.1f} band-pairs/s, "
-                f"{rec[prec]['frac_hbm'] * 100:.1f}% of HBM roofline")
+                f"{rec[prec]['frac_hbm'] * 100:.1f}% of HBM roofline, worst sampled rel L2 "
+                f"vs FP64 {err['max_rel_l2_vs_fp64']:.2e}")
             del trs, ins, outs
             torch.cuda.empty_cache()
         out[name] = rec
     return out
+
+
+def batch_error(sp, torch, np, trs, ins, outs, sets, bands, dims, prec):
+    """Worst relative L2 error of sampled bands of the batch just timed
+    (backward field and scaled forward values) against the library's FP64
+    single-transform path on the same inputs; that path matches the FP64
+    oracle to 1e-12 in the GPU parity tests. The batch runs once more with the
+    sampled bands' fields in separate buffers (others in their space domain)."""
+    n = dims[0]
+    ntr = len(trs)
+    sample = sorted({0, ntr // 3, (2 * ntr) // 3, ntr - 1})
+    real = torch.float64 if prec == "double" else torch.float32
+    fields = {k: torch.empty(2 * n ** 3, dtype=real, device="cuda") for k in sample}
+    spaces = [fields[k].data_ptr() if k in fields else trs[k].space_domain_data(sp.SPFFT_PU_GPU)
+              for k in range(ntr)]
+    sp.multi_transform_backward(trs, ins, spaces)
+    sp.multi_transform_forward(trs, spaces, outs, [sp.SPFFT_FULL_SCALING] * ntr)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for k in sample:
+        t = sets[k // bands]
+        ref = sp.Transform(-1, sp.SPFFT_PU_GPU, sp.SPFFT_TRANS_C2C, *dims, len(t),
+                           sp.SPFFT_INDEX_TRIPLETS, t)
+        vin = ins[k].to(torch.float64)
+        field = torch.empty(2 * n ** 3, dtype=torch.float64, device="cuda")
+        ref.backward(vin.data_ptr(), field.data_ptr())
+        vout = torch.empty(2 * len(t), dtype=torch.float64, device="cuda")
+        ref.forward(field.data_ptr(), vout.data_ptr(), sp.SPFFT_FULL_SCALING)
+        torch.cuda.synchronize()
+        e_b = float(torch.linalg.vector_norm(fields[k].to(torch.float64) - field) /
+                    torch.linalg.vector_norm(field))
+        e_f = float(torch.linalg.vector_norm(outs[k].to(torch.float64) - vout) /
+                    torch.linalg.vector_norm(vout))
+        worst = max(worst, e_b, e_f)
+        del ref, field, vout
+    return {"max_rel_l2_vs_fp64": worst, "error_bands_sampled": sample}
 
 
 def launch_floor(torch, kernel_us):
@@ -368,6 +440,34 @@ def main():
     log(f"timed {args.steps} pairs in {elapsed * 1e3:.1f} ms -> {value:.1f} pairs/s "
         f"(host enqueue {t_enq / args.steps * 1e6:.1f} us/pair)")
 
+    # The API's default mode (SPFFT_EXEC_SYNCHRONOUS: every call waits for
+    # its work) on the same rotation, and a warm single transform (no
+    # rotation: its ~77 MB of buffers stay in the 256 MB Infinity Cache).
+    extra = {}
+    for k in range(args.rotate):
+        trs[k].set_execution_mode(sp.SPFFT_EXEC_SYNCHRONOUS)
+    nsync = max(10, args.steps // 4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(nsync):
+        pair(i)
+    torch.cuda.synchronize()
+    extra["plain_synchronous"] = {"pairs_per_s": nsync / (time.perf_counter() - t0),
+                                  "pairs": nsync, "rotate": args.rotate}
+    for k in range(args.rotate):
+        trs[k].set_execution_mode(sp.SPFFT_EXEC_ASYNCHRONOUS)
+    for i in range(args.warmup):
+        pair(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        pair(0)
+    torch.cuda.synchronize()
+    extra["plain_warm_cache"] = {"pairs_per_s": args.steps / (time.perf_counter() - t0),
+                                 "pairs": args.steps, "rotate": 1}
+    log(f"synchronous mode: {extra['plain_synchronous']['pairs_per_s']:.1f} pairs/s; warm "
+        f"single transform: {extra['plain_warm_cache']['pairs_per_s']:.1f} pairs/s")
+
     # Same steps again with per-kernel HIP events on the library's stream.
     sp.lib.spfft_amd_profiling_enable(1)
     for i in range(args.steps):
@@ -434,6 +534,7 @@ def main():
             "wall_frac": pair_bytes * value / 1e9 / HBM_PEAK_GBS,
         },
         "kernels": kernels,
+        **extra,
     }
     result["launch_floor"] = launch_floor(torch, d["mean_us"])
     result["plain_r2c"] = plain_r2c_bench(args, sp, torch, np)

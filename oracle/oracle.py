@@ -54,8 +54,40 @@ def lib() -> ctypes.CDLL:
         L.oracle_backward.argtypes = [ctypes.c_void_p, dp, dp]
         L.oracle_forward.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_int]
         L.oracle_fft1d.argtypes = [ctypes.c_int, dp, dp, ctypes.c_int]
+        L.oracle_use_fft_library.argtypes = [ctypes.c_char_p]
+        L.oracle_use_fft_library.restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+# Libraries exporting the FFTW3 C API the reference's host path calls
+# (src/fft/fftw_interface.hpp:41-101): FFTW itself, or MKL's FFTW wrappers.
+FFT_LIBRARIES = ["libfftw3.so.3", "/opt/conda/lib/libmkl_rt.so.1", "/opt/conda/lib/libmkl_rt.so"]
+_provider = "built-in mixed-radix FFT (oracle)"
+
+
+def use_fft_library(candidates=None):
+    """Route the 1D lines of plans created from now on through the first
+    loadable FFTW3-API library (for the CPU baseline timing). MKL runs with
+    MKL_THREADING_LAYER=SEQUENTIAL, as the reference's CMake prefers. Returns
+    the provider name, or None (built-in FFT kept)."""
+    global _provider
+    os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+    for path in candidates or FFT_LIBRARIES:
+        if lib().oracle_use_fft_library(path.encode()) == 0:
+            _provider = f"FFTW3 C API of {path}"
+            return _provider
+    return None
+
+
+def use_builtin_fft():
+    global _provider
+    lib().oracle_use_fft_library(None)
+    _provider = "built-in mixed-radix FFT (oracle)"
+
+
+def fft_provider() -> str:
+    return _provider
 
 
 def _iptr(a):

@@ -33,6 +33,7 @@
  * Repeated triplets: the reference's OpenMP scatter makes the surviving
  * value unspecified; this oracle keeps the last one (the serial order).
  */
+#include <dlfcn.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -124,6 +125,61 @@ static void fft_line(const fft_plan* p, const cplx* in, size_t stride, cplx* out
   ct_rec(in, stride, out, p->n, p->fac, p, 1, sign, tmp);
 }
 
+/* ------------------------------------------- FFTW3 C API, loaded at run time
+ *
+ * The reference's host path hands its 1D lines to FFTW 3
+ * (src/fft/fftw_interface.hpp:41-101, src/fft/fftw_plan_1d.hpp:86-139:
+ * fftw_plan_many_dft with FFTW_ESTIMATE, fftw_execute_dft on new arrays). For
+ * the CPU baseline timing the oracle can do the same through any library that
+ * exports that API (libfftw3.so.3, or MKL's libmkl_rt.so), opened with dlopen
+ * by oracle_use_fft_library(). It is off unless enabled: the parity checks use
+ * the built-in FFT above, and a CPU test compares the two providers. */
+typedef void* (*fftw_plan_many_fn)(int, const int*, int, cplx*, const int*, int, int, cplx*,
+                                   const int*, int, int, int, unsigned);
+typedef void (*fftw_execute_dft_fn)(void*, cplx*, cplx*);
+typedef void (*fftw_destroy_plan_fn)(void*);
+
+static struct {
+  void* handle;
+  fftw_plan_many_fn plan_many;
+  fftw_execute_dft_fn execute_dft;
+  fftw_destroy_plan_fn destroy_plan;
+} g_fftw;
+
+enum { FFTW_FORWARD_SIGN = -1, FFTW_BACKWARD_SIGN = 1 };
+#define FFTW_FLAG_UNALIGNED (1U << 1)
+#define FFTW_FLAG_ESTIMATE (1U << 6)
+
+/* 0 on success; later plans use the library. NULL turns it off again. */
+int oracle_use_fft_library(const char* path) {
+  if (!path) {
+    g_fftw.plan_many = NULL;
+    return 0;
+  }
+  void* h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+  if (!h) return 1;
+  fftw_plan_many_fn pm = (fftw_plan_many_fn)dlsym(h, "fftw_plan_many_dft");
+  fftw_execute_dft_fn ex = (fftw_execute_dft_fn)dlsym(h, "fftw_execute_dft");
+  fftw_destroy_plan_fn de = (fftw_destroy_plan_fn)dlsym(h, "fftw_destroy_plan");
+  if (!pm || !ex || !de) {
+    dlclose(h);
+    return 2;
+  }
+  g_fftw.handle = h;
+  g_fftw.plan_many = pm;
+  g_fftw.execute_dft = ex;
+  g_fftw.destroy_plan = de;
+  return 0;
+}
+
+/* One out-of-place line plan: length n, input stride, unit output stride. */
+static void* ext_plan(int n, int istride, int sign, cplx* in, cplx* out) {
+  if (!g_fftw.plan_many) return NULL;
+  const int dims[1] = {n};
+  return g_fftw.plan_many(1, dims, 1, in, NULL, istride, n * istride, out, NULL, 1, n, sign,
+                          FFTW_FLAG_ESTIMATE | FFTW_FLAG_UNALIGNED);
+}
+
 /* ------------------------------------------------------------ index map */
 
 static int cmp_int(const void* a, const void* b) {
@@ -183,7 +239,21 @@ typedef struct {
   cplx* sticks; /* [stick][z] */
   cplx* planes; /* [z][xFreq][y] */
   int maxlen;
+  void* ext[7]; /* FFTW3-API line plans (EXT_*), all NULL with the built-in FFT */
 } oracle_plan;
+
+enum { EXT_Z_BWD, EXT_Y_BWD, EXT_Y_FWD, EXT_X_BWD_STRIDED, EXT_X_BWD, EXT_X_FWD, EXT_Z_FWD };
+
+/* out = DFT of the strided line `in`, by the plan's FFTW3-API line plan
+ * `which` when there is one, else by the built-in FFT. */
+static void run_line(const oracle_plan* p, int which, const fft_plan* fp, const cplx* in,
+                     size_t stride, cplx* out, int sign, cplx* tmp) {
+  if (p->ext[which]) {
+    g_fftw.execute_dft(p->ext[which], (cplx*)in, out);
+  } else {
+    fft_line(fp, in, stride, out, sign, tmp);
+  }
+}
 
 void oracle_plan_destroy(oracle_plan* p) {
   if (!p) return;
@@ -195,6 +265,8 @@ void oracle_plan_destroy(oracle_plan* p) {
   fft_plan_free(&p->px);
   fft_plan_free(&p->py);
   fft_plan_free(&p->pz);
+  for (int k = 0; k < 7; ++k)
+    if (p->ext[k]) g_fftw.destroy_plan(p->ext[k]);
   free(p);
 }
 
@@ -238,6 +310,20 @@ oracle_plan* oracle_plan_create(int r2c, int nx, int ny, int nz, int n, const in
   p->planes = (cplx*)malloc(sizeof(cplx) * (size_t)nz * p->nxf * ny);
   p->maxlen = nx > ny ? nx : ny;
   if (nz > p->maxlen) p->maxlen = nz;
+  if (g_fftw.plan_many) {
+    /* planning only (FFTW_ESTIMATE reads no data); executed on new arrays */
+    cplx* a = (cplx*)calloc((size_t)nz * p->nxf * ny + 1, sizeof(cplx));
+    cplx* b = (cplx*)calloc((size_t)p->maxlen + 1, sizeof(cplx));
+    p->ext[EXT_Z_BWD] = ext_plan(nz, 1, FFTW_BACKWARD_SIGN, a, b);
+    p->ext[EXT_Y_BWD] = ext_plan(ny, 1, FFTW_BACKWARD_SIGN, a, b);
+    p->ext[EXT_Y_FWD] = ext_plan(ny, 1, FFTW_FORWARD_SIGN, a, b);
+    p->ext[EXT_X_BWD_STRIDED] = ext_plan(nx, ny, FFTW_BACKWARD_SIGN, a, b);
+    p->ext[EXT_X_BWD] = ext_plan(nx, 1, FFTW_BACKWARD_SIGN, a, b);
+    p->ext[EXT_X_FWD] = ext_plan(nx, 1, FFTW_FORWARD_SIGN, a, b);
+    p->ext[EXT_Z_FWD] = ext_plan(nz, p->nxf * ny, FFTW_FORWARD_SIGN, a, b);
+    free(a);
+    free(b);
+  }
   return p;
 }
 
@@ -293,7 +379,7 @@ void oracle_backward(oracle_plan* p, const double* values, double* out) {
     /* z transform, then transpose into planes[z][x][y] */
     ORACLE_FOR
     for (int s = 0; s < nst; ++s) {
-      fft_line(&p->pz, p->sticks + (size_t)s * nz, 1, line, +1, tmp);
+      run_line(p, EXT_Z_BWD, &p->pz, p->sticks + (size_t)s * nz, 1, line, +1, tmp);
       const int x = p->stickKey[s] / ny, y = p->stickKey[s] % ny;
       for (int z = 0; z < nz; ++z) p->planes[((size_t)z * nxf + x) * ny + y] = line[z];
     }
@@ -307,7 +393,7 @@ void oracle_backward(oracle_plan* p, const double* values, double* out) {
     for (long long zu = 0; zu < (long long)nz * p->nuniq; ++zu) {
       const int z = (int)(zu / p->nuniq), x = p->uniqX[zu % p->nuniq];
       cplx* col = p->planes + ((size_t)z * nxf + x) * ny;
-      fft_line(&p->py, col, 1, line, +1, tmp);
+      run_line(p, EXT_Y_BWD, &p->py, col, 1, line, +1, tmp);
       memcpy(col, line, sizeof(cplx) * (size_t)ny);
     }
     /* x transform of every row into the space domain */
@@ -317,13 +403,13 @@ void oracle_backward(oracle_plan* p, const double* values, double* out) {
       const int z = (int)(zy / ny), y = (int)(zy % ny);
       const cplx* src = p->planes + (size_t)z * nxf * ny + y;
       if (!p->r2c) {
-        fft_line(&p->px, src, (size_t)ny, (cplx*)out + (size_t)zy * nx, +1, tmp);
+        run_line(p, EXT_X_BWD_STRIDED, &p->px, src, (size_t)ny, (cplx*)out + (size_t)zy * nx, +1, tmp);
       } else {
         for (int k = 0; k < nxf; ++k) full[k] = src[(size_t)k * ny];
         full[0].im = 0.0;
         if (nx % 2 == 0) full[nx / 2].im = 0.0;
         for (int k = nxf; k < nx; ++k) { full[k].re = full[nx - k].re; full[k].im = -full[nx - k].im; }
-        fft_line(&p->px, full, 1, line, +1, tmp);
+        run_line(p, EXT_X_BWD, &p->px, full, 1, line, +1, tmp);
         for (int x = 0; x < nx; ++x) out[(size_t)zy * nx + x] = line[x].re;
       }
     }
@@ -347,10 +433,10 @@ void oracle_forward(oracle_plan* p, const double* space, double* out, int scalin
     for (long long zy = 0; zy < (long long)nz * ny; ++zy) {
       const int z = (int)(zy / ny), y = (int)(zy % ny);
       if (!p->r2c) {
-        fft_line(&p->px, (const cplx*)space + (size_t)zy * nx, 1, line, -1, tmp);
+        run_line(p, EXT_X_FWD, &p->px, (const cplx*)space + (size_t)zy * nx, 1, line, -1, tmp);
       } else {
         for (int x = 0; x < nx; ++x) { real[x].re = space[(size_t)zy * nx + x]; real[x].im = 0.0; }
-        fft_line(&p->px, real, 1, line, -1, tmp);
+        run_line(p, EXT_X_FWD, &p->px, real, 1, line, -1, tmp);
       }
       cplx* dst = p->planes + (size_t)z * nxf * ny + y;
       for (int k = 0; k < nxf; ++k) dst[(size_t)k * ny] = line[k];
@@ -360,14 +446,14 @@ void oracle_forward(oracle_plan* p, const double* space, double* out, int scalin
     for (long long zu = 0; zu < (long long)nz * p->nuniq; ++zu) {
       const int z = (int)(zu / p->nuniq), x = p->uniqX[zu % p->nuniq];
       cplx* col = p->planes + ((size_t)z * nxf + x) * ny;
-      fft_line(&p->py, col, 1, line, -1, tmp);
+      run_line(p, EXT_Y_FWD, &p->py, col, 1, line, -1, tmp);
       memcpy(col, line, sizeof(cplx) * (size_t)ny);
     }
     /* gather sticks and z transform */
     ORACLE_FOR
     for (int s = 0; s < nst; ++s) {
       const int x = p->stickKey[s] / ny, y = p->stickKey[s] % ny;
-      fft_line(&p->pz, p->planes + (size_t)x * ny + y, (size_t)nxf * ny,
+      run_line(p, EXT_Z_FWD, &p->pz, p->planes + (size_t)x * ny + y, (size_t)nxf * ny,
                p->sticks + (size_t)s * nz, -1, tmp);
     }
     free(real);

@@ -178,7 +178,8 @@ class _GridBase:
                                              transform_type, dim_x, dim_y, dim_z, local_z_length,
                                              num_local_elements, index_format, p),
                "transform_create")
-        return self._transform_cls._from_handle(h, self._precision)
+        return self._transform_cls._from_handle(h, self._precision, transform_type,
+                                                num_local_elements)
 
     def close(self):
         if getattr(self, "handle", None):
@@ -201,12 +202,19 @@ class _TransformBase:
             ctypes.byref(h), max_num_threads, processing_unit, transform_type, dim_x, dim_y,
             dim_z, num_local_elements, index_format, p), "transform_create_independent")
         self.handle = h
+        self.transform_type = transform_type
+        self.num_values = num_local_elements
 
     @classmethod
-    def _from_handle(cls, handle, precision):
+    def _from_handle(cls, handle, precision, transform_type, num_values):
         obj = cls.__new__(cls)
         obj._api = _API[precision]
         obj.handle = handle
+        # The C ABI has no accessor for the type, and its num_local_elements
+        # returns the local z length (reference transform.cpp:433-445), so
+        # both are kept from creation.
+        obj.transform_type = transform_type
+        obj.num_values = num_values
         return obj
 
     def _get(self, name, ctype=_i):
@@ -233,7 +241,7 @@ class _TransformBase:
     def clone(self):
         h = _vp()
         _check(self._api["transform_clone"](self.handle, ctypes.byref(h)), "transform_clone")
-        return type(self)._from_handle(h, self._precision)
+        return type(self)._from_handle(h, self._precision, self.transform_type, self.num_values)
 
     def space_domain_data(self, location):
         p = _vp()

@@ -13,7 +13,11 @@ between forward and backward: the backward recomputes it, so memory per band
 stays at one dense field regardless of the batch size.
 
 Tensors are complex128 (double transforms) or complex64 (float transforms) on
-the GPU; the dense field is shaped (Nz, Ny, Nx).
+the GPU; the dense field is shaped (Nz, Ny, Nx). Only C2C transforms qualify;
+shapes, dtypes, devices and the transform type are checked before any pointer
+reaches the library (ValueError otherwise). The library enqueues on the HIP
+default stream; when torch's current stream is another one, the default stream
+first waits for it and the current stream then waits for the transform.
 """
 from __future__ import annotations
 
@@ -26,20 +30,64 @@ def _dtype(transform):
     return torch.complex128 if transform._precision == "double" else torch.complex64
 
 
+def _check_tensor(name, x, transform, numel):
+    want = _dtype(transform)
+    if x.dtype != want:
+        raise ValueError(f"{name}: dtype {x.dtype}, the transform needs {want}")
+    if x.device.type != "cuda":
+        raise ValueError(f"{name}: tensor on {x.device}, the transform runs on the GPU")
+    if transform.device_id() != (x.device.index if x.device.index is not None
+                                 else torch.cuda.current_device()):
+        raise ValueError(f"{name}: tensor on {x.device}, the transform on device "
+                         f"{transform.device_id()}")
+    if x.numel() != numel:
+        raise ValueError(f"{name}: {x.numel()} elements, the transform needs {numel}")
+
+
+def _check_transform(transform, dims):
+    if transform.transform_type != sp.SPFFT_TRANS_C2C:
+        raise ValueError("only C2C transforms have a complex field to differentiate")
+    if tuple(dims) != (transform.dim_x(), transform.dim_y(), transform.dim_z()):
+        raise ValueError(f"dims {tuple(dims)} differ from the transform's "
+                         f"{(transform.dim_x(), transform.dim_y(), transform.dim_z())}")
+
+
+class _OnDefaultStream:
+    """Orders a library call (HIP default stream) against torch's current stream."""
+
+    def __enter__(self):
+        self.cur = torch.cuda.current_stream()
+        self.dflt = torch.cuda.default_stream()
+        self.other = self.cur.cuda_stream != self.dflt.cuda_stream
+        if self.other:
+            self.dflt.wait_stream(self.cur)
+
+    def __exit__(self, *exc):
+        if self.other:
+            self.cur.wait_stream(self.dflt)
+        return False
+
+
 def _backward(transform, dims, c):
     nx, ny, nz = dims
+    _check_transform(transform, dims)
+    _check_tensor("c", c, transform, transform.num_values)
     c = c.contiguous()
     psi = torch.empty((nz, ny, nx), dtype=c.dtype, device=c.device)
-    transform.backward(torch.view_as_real(c).data_ptr(), torch.view_as_real(psi).data_ptr())
+    with _OnDefaultStream():
+        transform.backward(torch.view_as_real(c).data_ptr(), torch.view_as_real(psi).data_ptr())
     return psi
 
 
 def _adjoint(transform, n, field):
     """B^H field: unnormalised forward restricted to the sparse set."""
+    _check_tensor("field", field, transform,
+                  transform.dim_x() * transform.dim_y() * transform.dim_z())
     field = field.contiguous()
     out = torch.empty(n, dtype=field.dtype, device=field.device)
-    transform.forward(torch.view_as_real(field).data_ptr(), torch.view_as_real(out).data_ptr(),
-                      sp.SPFFT_NO_SCALING)
+    with _OnDefaultStream():
+        transform.forward(torch.view_as_real(field).data_ptr(),
+                          torch.view_as_real(out).data_ptr(), sp.SPFFT_NO_SCALING)
     return out
 
 

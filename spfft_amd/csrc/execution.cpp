@@ -5,7 +5,9 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstdio>
+#include <cstring>
 #include <list>
+#include <unordered_map>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -37,6 +39,33 @@ std::pair<P*, P*> translate_pointer(P* ptr) {
   }
   return {attr.hostPointer ? static_cast<P*>(attr.hostPointer) : ptr, nullptr};
 }
+
+// A user pointer classified for the copy path: device memory is used in
+// place, pinned host memory is copied by DMA directly, pageable host memory
+// goes through the grid's pinned ring.
+enum class MemKind { Device, Pinned, Pageable };
+template <typename P>
+MemKind classify(P* ptr, P** view) {
+  hipPointerAttribute_t attr{};
+  const hipError_t status = hipPointerGetAttributes(&attr, static_cast<const void*>(ptr));
+  if (status != hipSuccess) {
+    (void)hipGetLastError();
+    if (status != hipErrorInvalidValue) check_hip(status);
+    *view = ptr;
+    return MemKind::Pageable;
+  }
+  if (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged) {
+    *view = static_cast<P*>(attr.devicePointer);
+    return MemKind::Device;
+  }
+  *view = attr.hostPointer ? static_cast<P*>(attr.hostPointer) : ptr;
+  return attr.type == hipMemoryTypeHost ? MemKind::Pinned : MemKind::Pageable;
+}
+
+// Host transfers are split into chunks of about this size (whole planes for
+// the space domain), so copies overlap the kernels and, for pageable memory,
+// the host-side copies overlap the DMA.
+constexpr size_t kIoChunkBytes = size_t(4) << 20;
 
 // SPFFT_AMD_GENERIC=1 forces the generic stage kernels (used to compare the
 // two paths).
@@ -116,79 +145,69 @@ void release_plan(BatchPlan& p) {
 
 }  // namespace
 
+// Device tables of one index set: built once per (device, precision, set,
+// path) and shared by every transform with an identical set, so bands over
+// one set hold, and a batch reads, one copy.
 template <typename T>
-ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
-                              const std::shared_ptr<GridInternal<T>>& grid)
-    : map_(map),
-      grid_(grid),
-      scale_(static_cast<T>(1.0 / static_cast<double>(static_cast<unsigned long long>(map->dimX) *
-                                                      static_cast<unsigned long long>(map->dimY) *
-                                                      static_cast<unsigned long long>(map->dimZ)))) {
-  serial_ = next_serial.fetch_add(1);
-  check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  check_hip(hipEventCreateWithFlags(&startEvent_, hipEventDisableTiming));
-  check_hip(hipEventCreateWithFlags(&endEvent_, hipEventDisableTiming));
+std::shared_ptr<TransformTables<T>> build_tables(const std::shared_ptr<IndexMap>& map, bool fast,
+                                                 int device, hipStream_t stream) {
+  auto tabPtr = std::make_shared<TransformTables<T>>();
+  TransformTables<T>& tb = *tabPtr;
+  tb.map = map;
+  tb.device = device;
+  tb.allowFast = fast;
+  const StickTables t = build_stick_tables(*map);
+  tb.numCols = static_cast<int>(t.colX.size());
+  // fixed-length kernels when the lengths have compile-time schedules
+  tb.fast = fast && fast_path_supported<T>(map->dimX, map->dimY, map->dimZ, tb.numCols,
+                                           map->type == SPFFT_TRANS_R2C);
+  const bool useFast = tb.fast;
+  tb.entryValue = DeviceBuffer<int>::from_host(t.entryValue);
+  tb.entrySlot = DeviceBuffer<int>::from_host(t.entrySlot);
+  tb.stickBegin = DeviceBuffer<int>::from_host(t.stickBegin);
+  tb.stickY = DeviceBuffer<int>::from_host(t.stickY);
+  tb.stickCol = DeviceBuffer<int>::from_host(t.stickCol);
+  tb.colX = DeviceBuffer<int>::from_host(t.colX);
+  tb.colBegin = DeviceBuffer<int>::from_host(t.colBegin);
 
-  const StickTables t = build_stick_tables(*map_);
-  numCols_ = static_cast<int>(t.colX.size());
-  entryValue_ = DeviceBuffer<int>::from_host(t.entryValue);
-  entrySlot_ = DeviceBuffer<int>::from_host(t.entrySlot);
-  stickBegin_ = DeviceBuffer<int>::from_host(t.stickBegin);
-  stickY_ = DeviceBuffer<int>::from_host(t.stickY);
-  stickCol_ = DeviceBuffer<int>::from_host(t.stickCol);
-  colX_ = DeviceBuffer<int>::from_host(t.colX);
-  colBegin_ = DeviceBuffer<int>::from_host(t.colBegin);
-
-  fftX_ = make_fft_desc(map_->dimX);
-  fftY_ = make_fft_desc(map_->dimY);
-  fftZ_ = make_fft_desc(map_->dimZ);
-  size_t lds = 0;
-  zBatch_ = stage_batch<T>(map_->dimZ, std::max(1, map_->num_sticks()), &lds);
-  yBatch_ = stage_batch<T>(map_->dimY, std::max(1, numCols_), &lds);
-  xBatch_ = stage_batch<T>(map_->dimX, map_->dimY, &lds);
-
-  // Fixed-length kernels when the lengths have compile-time schedules.
-  fast_ = !disable_fast_path() &&
-          fast_path_supported<T>(map_->dimX, map_->dimY, map_->dimZ, numCols_,
-                                 map_->type == SPFFT_TRANS_R2C);
-  if (fast_) {
+  if (useFast) {
     // Stick index at (column, y); absent positions, and the extra row
-    // `numCols_` used by the lanes past the last column, point at the zero
+    // `tb.numCols` used by the lanes past the last column, point at the zero
     // element that every row of the [z][stick] buffer holds at index nst.
     // Stored class-interleaved: for column u and j < M = Ny / R the R entries
     // of y = j + M l (l < R) are adjacent, so a lane of the backward plane
     // kernel reads all the indices of one decimation-in-frequency output with
     // R / 4 16-byte loads.
-    const int ny = map_->dimY;
-    const int R = fast_classes(map_->dimX), M = ny / R;
-    std::vector<int> at(static_cast<size_t>(numCols_ + 1) * ny, map_->num_sticks());
-    for (int s = 0; s < map_->num_sticks(); ++s) {
+    const int ny = map->dimY;
+    const int R = fast_classes(map->dimX), M = ny / R;
+    std::vector<int> at(static_cast<size_t>(tb.numCols + 1) * ny, map->num_sticks());
+    for (int s = 0; s < map->num_sticks(); ++s) {
       const int y = t.stickY[s];
       at[(static_cast<size_t>(t.stickCol[s]) * M + y % M) * R + y / M] = s;
     }
-    stickAt_ = DeviceBuffer<int>::from_host(at);
+    tb.stickAt = DeviceBuffer<int>::from_host(at);
     // Entry list for the z kernels: value index and position in the LDS tile
     // of the workgroup that owns the stick; earlier copies of a repeated
     // triplet are flagged so that the last one wins on scatter.
     int S = 0, LZ = 0, shift = 0;
-    fast_z_geometry<T>(map_->dimZ, &S, &LZ, &shift);
-    std::vector<int2> ent(map_->num_values());
-    for (int e = 0; e < map_->num_values(); ++e) {
+    fast_z_geometry<T>(map->dimZ, &S, &LZ, &shift);
+    std::vector<int2> ent(map->num_values());
+    for (int e = 0; e < map->num_values(); ++e) {
       const int slot = t.entrySlot[e];
-      const int st = slot / map_->dimZ, z = slot % map_->dimZ;
+      const int st = slot / map->dimZ, z = slot % map->dimZ;
       int pos = (st % S) * LZ + z + (z >> shift);
-      if (e + 1 < map_->num_values() && t.entrySlot[e + 1] == slot) pos |= int(0x80000000u);
+      if (e + 1 < map->num_values() && t.entrySlot[e + 1] == slot) pos |= int(0x80000000u);
       ent[e] = make_int2(t.entryValue[e], pos);
     }
-    zEntries_ = DeviceBuffer<int2>::from_host(ent);
+    tb.zEntries = DeviceBuffer<int2>::from_host(ent);
     // Sticks whose values form one run in user order: storage z0, z0 + 1, ...
     // (cyclically, e.g. centred z ascending) at consecutive value indices.
     // When every stick is such a run the z kernels address values directly.
     {
-      const int nz = map_->dimZ;
-      std::vector<int4> runs(map_->num_sticks());
-      bool all = map_->num_sticks() > 0;
-      for (int st = 0; st < map_->num_sticks() && all; ++st) {
+      const int nz = map->dimZ;
+      std::vector<int4> runs(map->num_sticks());
+      bool all = map->num_sticks() > 0;
+      for (int st = 0; st < map->num_sticks() && all; ++st) {
         const int b = t.stickBegin[st], e = t.stickBegin[st + 1];
         const int len = e - b;
         // storage z of the entries ascend (sorted by slot); one run has at
@@ -212,14 +231,14 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
         }
         runs[st] = make_int4(v0, z0, len, 0);
       }
-      if (all) zInfo_ = DeviceBuffer<int4>::from_host(runs);
+      if (all) tb.zInfo = DeviceBuffer<int4>::from_host(runs);
     }
-    const int classes = fast_classes(map_->dimX);
+    const int classes = fast_classes(map->dimX);
     std::vector<int> resBegin(classes + 1, 0), resSticks;
-    resSticks.reserve(map_->num_sticks());
+    resSticks.reserve(map->num_sticks());
     for (int c = 0; c < classes; ++c) {
       resBegin[c] = static_cast<int>(resSticks.size());
-      for (int s = 0; s < map_->num_sticks(); ++s) {
+      for (int s = 0; s < map->num_sticks(); ++s) {
         if (t.colX[t.stickCol[s]] % classes == c) resSticks.push_back(s);
       }
     }
@@ -229,31 +248,110 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
       const int st = resSticks[i];
       resYM[i] = (t.stickY[st] << 16) | (t.colX[t.stickCol[st]] / classes);
     }
-    resYM_ = DeviceBuffer<int>::from_host(resYM);
-    // column of x, or the all-zero column numCols_ of the xy tile
-    std::vector<int> xCol(map_->dimX, numCols_);
-    for (int u = 0; u < numCols_; ++u) xCol[t.colX[u]] = u;
-    xCol_ = DeviceBuffer<int>::from_host(xCol);
-    resSticks_ = DeviceBuffer<int>::from_host(resSticks);
-    resBegin_ = DeviceBuffer<int>::from_host(resBegin);
+    tb.resYM = DeviceBuffer<int>::from_host(resYM);
+    // column of x, or the all-zero column tb.numCols of the xy tile
+    std::vector<int> xCol(map->dimX, tb.numCols);
+    for (int u = 0; u < tb.numCols; ++u) xCol[t.colX[u]] = u;
+    tb.xCol = DeviceBuffer<int>::from_host(xCol);
+    tb.resSticks = DeviceBuffer<int>::from_host(resSticks);
+    tb.resBegin = DeviceBuffer<int>::from_host(resBegin);
     // tile pitch for the whole-plane batch kernels (0: not applicable)
-    if (plane_path_supported<T>(map_->dimX, numCols_, map_->type == SPFFT_TRANS_R2C)) {
-      pitchT_ = plane_pitch<T>(map_->dimX, numCols_);
+    if (plane_path_supported<T>(map->dimX, tb.numCols, map->type == SPFFT_TRANS_R2C)) {
+      tb.pitchT = plane_pitch<T>(map->dimX, tb.numCols);
     }
   }
 
-  twX_ = DeviceBuffer<Cx<T>>(map_->dimX);
-  twY_ = DeviceBuffer<Cx<T>>(map_->dimY);
-  twZ_ = DeviceBuffer<Cx<T>>(map_->dimZ);
-  generate_twiddles<T>(twX_.data(), map_->dimX, stream_);
-  generate_twiddles<T>(twY_.data(), map_->dimY, stream_);
-  generate_twiddles<T>(twZ_.data(), map_->dimZ, stream_);
-  check_hip(hipStreamSynchronize(stream_));
+  tb.twX = DeviceBuffer<Cx<T>>(map->dimX);
+  tb.twY = DeviceBuffer<Cx<T>>(map->dimY);
+  tb.twZ = DeviceBuffer<Cx<T>>(map->dimZ);
+  generate_twiddles<T>(tb.twX.data(), map->dimX, stream);
+  generate_twiddles<T>(tb.twY.data(), map->dimY, stream);
+  generate_twiddles<T>(tb.twZ.data(), map->dimZ, stream);
+  check_hip(hipStreamSynchronize(stream));
+  return tabPtr;
+}
+
+
+// Hash of everything the tables depend on.
+std::uint64_t table_key(const IndexMap& m, bool fast, int device, size_t realBytes) {
+  std::uint64_t h = 1469598103934665603ull;
+  auto mix = [&h](std::uint64_t v) {
+    h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+  };
+  mix(static_cast<std::uint64_t>(m.dimX));
+  mix(static_cast<std::uint64_t>(m.dimY));
+  mix(static_cast<std::uint64_t>(m.dimZ));
+  mix(static_cast<std::uint64_t>(m.type));
+  mix(fast ? 1 : 0);
+  mix(static_cast<std::uint64_t>(device));
+  mix(realBytes);
+  for (int v : m.valueSlot) mix(static_cast<std::uint32_t>(v));
+  for (int v : m.stickKey) mix(static_cast<std::uint32_t>(v));
+  return h;
+}
+
+bool same_set(const IndexMap& a, const IndexMap& b) {
+  return a.dimX == b.dimX && a.dimY == b.dimY && a.dimZ == b.dimZ && a.type == b.type &&
+         a.valueSlot == b.valueSlot && a.stickKey == b.stickKey;
+}
+
+template <typename T>
+std::shared_ptr<TransformTables<T>> acquire_tables(const std::shared_ptr<IndexMap>& map,
+                                                   bool fast, int device, hipStream_t stream) {
+  static std::mutex mutex;
+  static auto* registry =
+      new std::unordered_multimap<std::uint64_t, std::weak_ptr<TransformTables<T>>>();
+  const std::uint64_t key = table_key(*map, fast, device, sizeof(T));
+  std::lock_guard<std::mutex> lock(mutex);
+  auto range = registry->equal_range(key);
+  for (auto it = range.first; it != range.second;) {
+    std::shared_ptr<TransformTables<T>> t = it->second.lock();
+    if (!t) {
+      it = registry->erase(it);
+      continue;
+    }
+    if (t->device == device && t->allowFast == fast && same_set(*t->map, *map)) return t;
+    ++it;
+  }
+  std::shared_ptr<TransformTables<T>> t = build_tables<T>(map, fast, device, stream);
+  registry->emplace(key, t);
+  return t;
+}
+
+template <typename T>
+ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
+                              const std::shared_ptr<GridInternal<T>>& grid)
+    : map_(map),
+      grid_(grid),
+      scale_(static_cast<T>(1.0 / static_cast<double>(static_cast<unsigned long long>(map->dimX) *
+                                                      static_cast<unsigned long long>(map->dimY) *
+                                                      static_cast<unsigned long long>(map->dimZ)))) {
+  serial_ = next_serial.fetch_add(1);
+  check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  check_hip(hipEventCreateWithFlags(&startEvent_, hipEventDisableTiming));
+  check_hip(hipEventCreateWithFlags(&endEvent_, hipEventDisableTiming));
+
+  const bool fast = !disable_fast_path();
+  tab_ = acquire_tables<T>(map_, fast, grid_->device_id(), stream_);
+  numCols_ = tab_->numCols;
+  fast_ = tab_->fast;
+  pitchT_ = tab_->pitchT;
+
+  fftX_ = make_fft_desc(map_->dimX);
+  fftY_ = make_fft_desc(map_->dimY);
+  fftZ_ = make_fft_desc(map_->dimZ);
+  size_t lds = 0;
+  zBatch_ = stage_batch<T>(map_->dimZ, std::max(1, map_->num_sticks()), &lds);
+  yBatch_ = stage_batch<T>(map_->dimY, std::max(1, numCols_), &lds);
+  xBatch_ = stage_batch<T>(map_->dimX, map_->dimY, &lds);
 }
 
 template <typename T>
 ExecutionGPU<T>::~ExecutionGPU() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  for (hipEvent_t e : ioEvents_) {
+    if (e) (void)hipEventDestroy(e);
+  }
   if (startEvent_) (void)hipEventDestroy(startEvent_);
   if (endEvent_) (void)hipEventDestroy(endEvent_);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -287,13 +385,13 @@ StageArgs<T> ExecutionGPU<T>::base_args() const {
   a.ncol = numCols_;
   a.r2c = map_->type == SPFFT_TRANS_R2C ? 1 : 0;
   a.zeroZeroStick = map_->zeroZeroStick;
-  a.entryValue = entryValue_.data();
-  a.entrySlot = entrySlot_.data();
-  a.stickBegin = stickBegin_.data();
-  a.stickY = stickY_.data();
-  a.stickCol = stickCol_.data();
-  a.colX = colX_.data();
-  a.colBegin = colBegin_.data();
+  a.entryValue = tab_->entryValue.data();
+  a.entrySlot = tab_->entrySlot.data();
+  a.stickBegin = tab_->stickBegin.data();
+  a.stickY = tab_->stickY.data();
+  a.stickCol = tab_->stickCol.data();
+  a.colX = tab_->colX.data();
+  a.colBegin = tab_->colBegin.data();
   a.zbuf = reinterpret_cast<Cx<T>*>(grid_->sticks_gpu());
   a.ybuf = reinterpret_cast<Cx<T>*>(grid_->work_gpu());
   a.scale = scale_;
@@ -307,80 +405,200 @@ FastArgs<T> ExecutionGPU<T>::fast_args() const {
   a.nst = map_->num_sticks();
   a.ncol = numCols_;
   a.zbuf = reinterpret_cast<Cx<T>*>(grid_->sticks_gpu());
-  a.entryValue = entryValue_.data();
-  a.entrySlot = entrySlot_.data();
-  a.stickBegin = stickBegin_.data();
-  a.stickY = stickY_.data();
-  a.stickCol = stickCol_.data();
-  a.colX = colX_.data();
-  a.colBegin = colBegin_.data();
-  a.stickAt = stickAt_.data();
-  a.zEntries = zEntries_.data();
+  a.entryValue = tab_->entryValue.data();
+  a.entrySlot = tab_->entrySlot.data();
+  a.stickBegin = tab_->stickBegin.data();
+  a.stickY = tab_->stickY.data();
+  a.stickCol = tab_->stickCol.data();
+  a.colX = tab_->colX.data();
+  a.colBegin = tab_->colBegin.data();
+  a.stickAt = tab_->stickAt.data();
+  a.zEntries = tab_->zEntries.data();
   a.pitch = fast_pitch<T>(map_->num_sticks());
   a.r2c = map_->type == SPFFT_TRANS_R2C ? 1 : 0;
   a.zeroZeroStick = map_->zeroZeroStick;
-  a.resSticks = resSticks_.data();
-  a.resBegin = resBegin_.data();
-  a.xCol = xCol_.data();
-  a.zInfo = zInfo_.data();
+  a.resSticks = tab_->resSticks.data();
+  a.resBegin = tab_->resBegin.data();
+  a.xCol = tab_->xCol.data();
+  a.zInfo = tab_->zInfo.data();
+  a.hasRuns = tab_->zInfo.data() != nullptr ? 1 : 0;
   a.pitchT = pitchT_;
-  a.resYM = resYM_.data();
-  a.twX = twX_.data();
-  a.twY = twY_.data();
-  a.twZ = twZ_.data();
+  a.resYM = tab_->resYM.data();
+  a.twX = tab_->twX.data();
+  a.twY = tab_->twY.data();
+  a.twZ = tab_->twZ.data();
   a.scale = scale_;
   return a;
 }
 
 template <typename T>
+bool ExecutionGPU<T>::device_accessible(const void* ptr) {
+  const void* view = nullptr;
+  return classify(ptr, &view) == MemKind::Device;
+}
+
+template <typename T>
+void ExecutionGPU<T>::io_prepare(size_t slotBytes) {
+  if (!ioEvents_[0]) {
+    for (hipEvent_t& e : ioEvents_) check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  ring_ = static_cast<char*>(grid_->host_staging(2 * slotBytes));
+  slotBytes_ = slotBytes;
+}
+
+template <typename T>
+int ExecutionGPU<T>::planes_per_chunk(size_t planeBytes) const {
+  const int nz = map_->dimZ;
+  int ppc = static_cast<int>(std::max<size_t>(1, kIoChunkBytes / std::max<size_t>(1, planeBytes)));
+  ppc = std::max(ppc, (nz + kIoMaxChunks - 1) / kIoMaxChunks);
+  return std::min(ppc, std::max(1, nz));
+}
+
+template <typename T>
+void ExecutionGPU<T>::upload_piece(char* dst, const char* src, size_t bytes, bool pinned, int k,
+                                   hipStream_t copy) {
+  hipEvent_t* done = &ioEvents_[kIoMaxChunks];
+  if (pinned) {
+    check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, copy));
+  } else {
+    // the slot is free once the DMA of the piece two earlier has read it
+    char* r = ring_ + (k & 1) * slotBytes_;
+    if (k >= 2) check_hip(hipEventSynchronize(done[(k - 2) % kIoMaxChunks]));
+    std::memcpy(r, src, bytes);
+    check_hip(hipMemcpyAsync(dst, r, bytes, hipMemcpyHostToDevice, copy));
+  }
+  check_hip(hipEventRecord(done[k % kIoMaxChunks], copy));
+}
+
+template <typename T>
+void ExecutionGPU<T>::upload(char* dst, const char* src, size_t bytes, bool pinned,
+                             hipStream_t compute, hipStream_t copy) {
+  hipEvent_t* done = &ioEvents_[kIoMaxChunks];
+  const int pieces = static_cast<int>((bytes + slotBytes_ - 1) / slotBytes_);
+  for (int k = 0; k < pieces; ++k) {
+    const size_t off = static_cast<size_t>(k) * slotBytes_;
+    upload_piece(dst + off, src + off, std::min(slotBytes_, bytes - off), pinned, k, copy);
+  }
+  if (pieces > 0 && copy != compute) {
+    check_hip(hipStreamWaitEvent(compute, done[(pieces - 1) % kIoMaxChunks], 0));
+  }
+  // the ring is free again when the call returns (a later call refills it)
+  if (pieces > 0 && !pinned) check_hip(hipEventSynchronize(done[(pieces - 1) % kIoMaxChunks]));
+}
+
+template <typename T>
+void ExecutionGPU<T>::download(char* dst, const char* src, size_t bytes, size_t pieceBytes,
+                               bool pinned, const hipEvent_t* ready, int readyCount,
+                               hipStream_t compute, hipStream_t copy) {
+  hipEvent_t* done = &ioEvents_[kIoMaxChunks];
+  const int pieces = static_cast<int>((bytes + pieceBytes - 1) / pieceBytes);
+  auto range = [&](int k, size_t* off, size_t* len) {
+    *off = static_cast<size_t>(k) * pieceBytes;
+    *len = std::min(pieceBytes, bytes - *off);
+  };
+  auto drain = [&](int k) {
+    size_t off = 0, len = 0;
+    range(k, &off, &len);
+    check_hip(hipEventSynchronize(done[k % kIoMaxChunks]));
+    std::memcpy(dst + off, ring_ + (k & 1) * slotBytes_, len);
+  };
+  for (int k = 0; k < pieces; ++k) {
+    size_t off = 0, len = 0;
+    range(k, &off, &len);
+    if (copy != compute) {
+      check_hip(hipStreamWaitEvent(copy, ready[std::min(k, readyCount - 1)], 0));
+    }
+    if (pinned) {
+      check_hip(hipMemcpyAsync(dst + off, src + off, len, hipMemcpyDeviceToHost, copy));
+    } else {
+      if (k >= 2) drain(k - 2);  // frees the slot
+      check_hip(hipMemcpyAsync(ring_ + (k & 1) * slotBytes_, src + off, len,
+                               hipMemcpyDeviceToHost, copy));
+      check_hip(hipEventRecord(done[k % kIoMaxChunks], copy));
+    }
+  }
+  if (pinned) {
+    if (copy != compute && pieces > 0) {
+      check_hip(hipEventRecord(done[0], copy));
+      check_hip(hipStreamWaitEvent(compute, done[0], 0));
+    }
+  } else {
+    for (int k = std::max(0, pieces - 2); k < pieces; ++k) drain(k);
+  }
+}
+
+template <typename T>
 void ExecutionGPU<T>::backward(const T* input, T* output, bool concurrent) {
   hipStream_t stream = begin(concurrent);
-  const size_t numValueReals = 2 * static_cast<size_t>(map_->num_values());
-  auto in = translate_pointer(input);
-  auto out = translate_pointer(output);
-  const T* inGPU = in.second;
-  if (numValueReals > 0 && !inGPU) {
-    // Host input: stage in the work buffer, which is free until the y stage.
-    check_hip(hipMemcpyAsync(grid_->work_gpu(), in.first, numValueReals * sizeof(T),
-                             hipMemcpyHostToDevice, stream));
+  // copies go to the transform's own stream, beside the kernels on the
+  // default stream (batched fallbacks run both on the transform's stream)
+  hipStream_t copy = concurrent ? stream : stream_;
+  const size_t valueBytes = 2 * static_cast<size_t>(map_->num_values()) * sizeof(T);
+  const T* inView = nullptr;
+  T* outView = nullptr;
+  const MemKind inKind = classify(input, &inView);
+  const MemKind outKind = classify(output, &outView);
+  const bool hostOut = outKind != MemKind::Device;
+  const size_t planeBytes = space_domain_size() / map_->dimZ * sizeof(T);
+  const int ppc = hostOut ? planes_per_chunk(planeBytes) : map_->dimZ;
+  const int chunks = (map_->dimZ + ppc - 1) / ppc;
+  if (inKind != MemKind::Device || hostOut) {
+    io_prepare(std::max(kIoChunkBytes, static_cast<size_t>(ppc) * planeBytes));
+    if (copy != stream) {
+      // the copy stream must not touch the buffers before earlier work ends
+      check_hip(hipEventRecord(startEvent_, stream));
+      check_hip(hipStreamWaitEvent(copy, startEvent_, 0));
+    }
+  }
+  const T* inGPU = inView;
+  if (valueBytes > 0 && inKind != MemKind::Device) {
+    // host values: staged into the work buffer, free until the plane stage
+    upload(reinterpret_cast<char*>(grid_->work_gpu()), reinterpret_cast<const char*>(inView),
+           valueBytes, inKind == MemKind::Pinned, stream, copy);
     inGPU = grid_->work_gpu();
   }
-  T* outGPU = out.second ? out.second : grid_->space_gpu();
+  T* outGPU = hostOut ? grid_->space_gpu() : outView;
+  hipEvent_t* ready = &ioEvents_[0];
 
   if (fast_) {
     FastArgs<T> f = fast_args();
     f.in = inGPU;
     f.out = outGPU;
     launch_fast_z_backward<T>(map_->dimZ, f, stream);
-    launch_fast_xy_backward<T>(map_->dimX, map_->dimY, f, stream);
-    if (out.first) {
-      check_hip(hipMemcpyAsync(out.first, outGPU, space_domain_size() * sizeof(T),
-                               hipMemcpyDeviceToHost, stream));
+    // host output: the plane stage runs in chunks of planes, each copied out
+    // while the next is computed
+    for (int c = 0; c < chunks; ++c) {
+      f.zBegin = c * ppc;
+      f.nz = std::min(map_->dimZ, (c + 1) * ppc);
+      launch_fast_xy_backward<T>(map_->dimX, map_->dimY, f, stream);
+      if (hostOut) check_hip(hipEventRecord(ready[c], stream));
     }
-    return;
+  } else {
+    StageArgs<T> a = base_args();
+    a.in = inGPU;
+    a.out = outGPU;
+    if (a.nst > 0) {
+      a.fft = fftZ_;
+      a.tw = tab_->twZ.data();
+      a.batch = zBatch_;
+      launch_z_backward<T>(a, stream);
+      a.fft = fftY_;
+      a.tw = tab_->twY.data();
+      a.batch = yBatch_;
+      launch_y_backward<T>(a, stream);
+    }
+    a.fft = fftX_;
+    a.tw = tab_->twX.data();
+    a.batch = xBatch_;
+    launch_x_backward<T>(a, stream);
+    if (hostOut) {
+      for (int c = 0; c < chunks; ++c) check_hip(hipEventRecord(ready[c], stream));
+    }
   }
-
-  StageArgs<T> a = base_args();
-  a.in = inGPU;
-  a.out = outGPU;
-  if (a.nst > 0) {
-    a.fft = fftZ_;
-    a.tw = twZ_.data();
-    a.batch = zBatch_;
-    launch_z_backward<T>(a, stream);
-    a.fft = fftY_;
-    a.tw = twY_.data();
-    a.batch = yBatch_;
-    launch_y_backward<T>(a, stream);
-  }
-  a.fft = fftX_;
-  a.tw = twX_.data();
-  a.batch = xBatch_;
-  launch_x_backward<T>(a, stream);
-
-  if (out.first) {
-    check_hip(hipMemcpyAsync(out.first, outGPU, space_domain_size() * sizeof(T),
-                             hipMemcpyDeviceToHost, stream));
+  if (hostOut) {
+    download(reinterpret_cast<char*>(outView), reinterpret_cast<const char*>(outGPU),
+             space_domain_size() * sizeof(T), static_cast<size_t>(ppc) * planeBytes,
+             outKind == MemKind::Pinned, ready, chunks, stream, copy);
   }
 }
 
@@ -388,54 +606,86 @@ template <typename T>
 void ExecutionGPU<T>::forward(const T* input, T* output, SpfftScalingType scaling,
                               bool concurrent) {
   hipStream_t stream = begin(concurrent);
-  const size_t numValueReals = 2 * static_cast<size_t>(map_->num_values());
-  auto in = translate_pointer(input);
-  auto out = translate_pointer(output);
-  const T* inGPU = in.second;
-  if (!inGPU) {
-    check_hip(hipMemcpyAsync(grid_->space_gpu(), in.first, space_domain_size() * sizeof(T),
-                             hipMemcpyHostToDevice, stream));
-    inGPU = grid_->space_gpu();
+  hipStream_t copy = concurrent ? stream : stream_;
+  const size_t valueBytes = 2 * static_cast<size_t>(map_->num_values()) * sizeof(T);
+  const T* inView = nullptr;
+  T* outView = nullptr;
+  const MemKind inKind = classify(input, &inView);
+  const MemKind outKind = classify(output, &outView);
+  if (valueBytes == 0) return;
+  const bool hostIn = inKind != MemKind::Device;
+  const size_t planeBytes = space_domain_size() / map_->dimZ * sizeof(T);
+  const int ppc = hostIn ? planes_per_chunk(planeBytes) : map_->dimZ;
+  const int chunks = (map_->dimZ + ppc - 1) / ppc;
+  const size_t chunkBytes = static_cast<size_t>(ppc) * planeBytes;
+  const size_t spaceBytes = space_domain_size() * sizeof(T);
+  if (hostIn || outKind != MemKind::Device) {
+    io_prepare(std::max(kIoChunkBytes, chunkBytes));
+    if (copy != stream) {
+      check_hip(hipEventRecord(startEvent_, stream));
+      check_hip(hipStreamWaitEvent(copy, startEvent_, 0));
+    }
   }
-  if (numValueReals == 0) return;
+  const T* inGPU = hostIn ? grid_->space_gpu() : inView;
   // Host output: the z stage writes into the work buffer, which is free again
-  // once the y stage has consumed it.
-  T* outGPU = out.second ? out.second : grid_->work_gpu();
+  // once the plane stage has consumed its input.
+  T* outGPU = outKind == MemKind::Device ? outView : grid_->work_gpu();
+  hipEvent_t* done = &ioEvents_[kIoMaxChunks];
+  auto upload_chunk = [&](int c) {
+    const size_t off = static_cast<size_t>(c) * chunkBytes;
+    upload_piece(reinterpret_cast<char*>(grid_->space_gpu()) + off,
+                 reinterpret_cast<const char*>(inView) + off, std::min(chunkBytes, spaceBytes - off),
+                 inKind == MemKind::Pinned, c, copy);
+    if (copy != stream) check_hip(hipStreamWaitEvent(stream, done[c % kIoMaxChunks], 0));
+  };
 
   if (fast_) {
     FastArgs<T> f = fast_args();
     f.in = inGPU;
     f.out = outGPU;
     f.useScale = scaling == SPFFT_FULL_SCALING ? 1 : 0;
-    launch_fast_xy_forward<T>(map_->dimX, map_->dimY, f, stream);
-    launch_fast_z_forward<T>(map_->dimZ, f, stream);
-    if (out.first) {
-      check_hip(hipMemcpyAsync(out.first, outGPU, numValueReals * sizeof(T),
-                               hipMemcpyDeviceToHost, stream));
+    // host input: each chunk of planes is transformed as soon as it arrives
+    for (int c = 0; c < chunks; ++c) {
+      if (hostIn) upload_chunk(c);
+      f.zBegin = c * ppc;
+      f.nz = std::min(map_->dimZ, (c + 1) * ppc);
+      launch_fast_xy_forward<T>(map_->dimX, map_->dimY, f, stream);
     }
-    return;
+    f.zBegin = 0;
+    f.nz = map_->dimZ;
+    launch_fast_z_forward<T>(map_->dimZ, f, stream);
+  } else {
+    if (hostIn) {
+      for (int c = 0; c < chunks; ++c) upload_chunk(c);
+    }
   }
-
-  StageArgs<T> a = base_args();
-  a.in = inGPU;
-  a.out = outGPU;
-  a.useScale = scaling == SPFFT_FULL_SCALING ? 1 : 0;
-  a.fft = fftX_;
-  a.tw = twX_.data();
-  a.batch = xBatch_;
-  launch_x_forward<T>(a, stream);
-  a.fft = fftY_;
-  a.tw = twY_.data();
-  a.batch = yBatch_;
-  launch_y_forward<T>(a, stream);
-  a.fft = fftZ_;
-  a.tw = twZ_.data();
-  a.batch = zBatch_;
-  launch_z_forward<T>(a, stream);
-
-  if (out.first) {
-    check_hip(hipMemcpyAsync(out.first, outGPU, numValueReals * sizeof(T), hipMemcpyDeviceToHost,
-                             stream));
+  if (hostIn && inKind == MemKind::Pageable) {
+    // the ring is free again when the call returns (a later call refills it)
+    check_hip(hipEventSynchronize(done[(chunks - 1) % kIoMaxChunks]));
+  }
+  if (!fast_) {
+    StageArgs<T> a = base_args();
+    a.in = inGPU;
+    a.out = outGPU;
+    a.useScale = scaling == SPFFT_FULL_SCALING ? 1 : 0;
+    a.fft = fftX_;
+    a.tw = tab_->twX.data();
+    a.batch = xBatch_;
+    launch_x_forward<T>(a, stream);
+    a.fft = fftY_;
+    a.tw = tab_->twY.data();
+    a.batch = yBatch_;
+    launch_y_forward<T>(a, stream);
+    a.fft = fftZ_;
+    a.tw = tab_->twZ.data();
+    a.batch = zBatch_;
+    launch_z_forward<T>(a, stream);
+  }
+  if (outKind != MemKind::Device) {
+    hipEvent_t* ready = &ioEvents_[0];
+    check_hip(hipEventRecord(ready[0], stream));
+    download(reinterpret_cast<char*>(outView), reinterpret_cast<const char*>(outGPU), valueBytes,
+             slotBytes_, outKind == MemKind::Pinned, ready, 1, stream, copy);
   }
 }
 
@@ -454,21 +704,19 @@ void ExecutionGPU<T>::synchronize(SpfftExecType mode, bool concurrent) {
 }
 
 template <typename T>
-bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
-                                const std::vector<const T*>& in, const std::vector<T*>& out,
-                                bool forward, const std::vector<SpfftScalingType>& scaling) {
-  const size_t n = ex.size();
+bool ExecutionGPU<T>::batch_qualifies(ExecutionGPU<T>* const* ex, size_t n, const T* const* in,
+                                      T* const* out, const T** din, T** dout,
+                                      BatchShape* shape) {
   if (n < 2) return false;
   const IndexMap& m0 = *ex[0]->map_;
-  for (const ExecutionGPU<T>* e : ex) {
-    const IndexMap& m = *e->map_;
-    if (!e->fast_ || m.dimX != m0.dimX || m.dimY != m0.dimY || m.dimZ != m0.dimZ ||
-        m.type != m0.type || m.num_values() == 0 || e->grid_->device_id() != ex[0]->grid_->device_id()) {
+  for (size_t t = 0; t < n; ++t) {
+    const IndexMap& m = *ex[t]->map_;
+    if (!ex[t]->fast_ || m.dimX != m0.dimX || m.dimY != m0.dimY || m.dimZ != m0.dimZ ||
+        m.type != m0.type || m.num_values() == 0 ||
+        ex[t]->grid_->device_id() != ex[0]->grid_->device_id()) {
       return false;
     }
   }
-  std::vector<const T*> din(n);
-  std::vector<T*> dout(n);
   for (size_t t = 0; t < n; ++t) {
     auto i = translate_pointer(in[t]);
     auto o = translate_pointer(out[t]);
@@ -476,7 +724,101 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
     din[t] = i.second;
     dout[t] = o.second;
   }
-  if (hipGetLastError() != hipSuccess) throw GPUPrecedingError();
+  int S = 0, LZ = 0, shift = 0;
+  fast_z_geometry<T>(m0.dimZ, &S, &LZ, &shift);
+  shape->blocks = 0;
+  shape->maxNcol = 0;
+  for (size_t t = 0; t < n; ++t) {
+    shape->blocks += (ex[t]->map_->num_sticks() + S - 1) / S;
+    shape->maxNcol = std::max(shape->maxNcol, ex[t]->numCols_);
+  }
+  shape->r2c = m0.type == SPFFT_TRANS_R2C;
+  shape->dimX = m0.dimX;
+  shape->dimZ = m0.dimZ;
+  return true;
+}
+
+template <typename T>
+bool ExecutionGPU<T>::plane_candidate(ExecutionGPU<T>* const* ex, size_t n, const T* const* din,
+                                      T* const* dout, bool forward, const BatchShape& shape) {
+  // whole-plane kernels are a candidate when every transform has its tile
+  // positions and the widest tile fits
+  bool ok = plane_kernels_forced() != 0 &&
+            plane_path_supported<T>(shape.dimX, shape.maxNcol, shape.r2c);
+  for (size_t t = 0; t < n && ok; ++t) ok = ex[t]->pitchT_ > 0;
+  // The FP32 whole-plane kernels move two elements per 16-byte access, so
+  // they need 16-byte aligned space-domain buffers; a complex64 view at an odd
+  // element offset (8-byte aligned) takes the class-split kernels instead.
+  for (size_t t = 0; t < n && ok; ++t) {
+    const auto space = reinterpret_cast<std::uintptr_t>(forward ? din[t] : dout[t]);
+    ok = space % 16 == 0;
+  }
+  return ok;
+}
+
+template <typename T>
+void ExecutionGPU<T>::fill_batch_tables(ExecutionGPU<T>* const* ex, size_t n, const T* const* din,
+                                        T* const* dout, bool forward,
+                                        const SpfftScalingType* scaling, FastArgs<T>* args,
+                                        int2* zmap) {
+  int S = 0, LZ = 0, shift = 0;
+  fast_z_geometry<T>(ex[0]->map_->dimZ, &S, &LZ, &shift);
+  size_t k = 0;
+  for (size_t t = 0; t < n; ++t) {
+    FastArgs<T> f = ex[t]->fast_args();
+    f.in = din[t];
+    f.out = dout[t];
+    f.useScale = forward && scaling[t] == SPFFT_FULL_SCALING ? 1 : 0;
+    f.pitchA = fast_xy_pitch_a<T>(ex[0]->map_->dimX, f.ncol);
+    const int b = (f.nst + S - 1) / S;
+    for (int j = 0; j < b; ++j) zmap[k++] = make_int2(static_cast<int>(t), j);
+    args[t] = f;
+  }
+}
+
+template <typename T>
+void ExecutionGPU<T>::launch_batch(bool forward, const BatchShape& shape, int n, bool plane,
+                                   const FastArgs<T>* dArgs, const int2* dMap,
+                                   hipStream_t stream) {
+  auto launch_xy = [&] {
+    if (plane) {
+      launch_plane_xy_batch<T>(forward, shape.dimX, dArgs, n, shape.dimZ, shape.maxNcol, stream);
+    } else {
+      launch_fast_xy_batch<T>(forward, shape.r2c, shape.dimX, dArgs, n, shape.dimZ,
+                              shape.maxNcol, stream);
+    }
+  };
+  if (forward) {
+    launch_xy();
+    launch_fast_z_batch<T>(true, shape.r2c, shape.dimZ, dArgs, dMap, shape.blocks, stream);
+  } else {
+    launch_fast_z_batch<T>(false, shape.r2c, shape.dimZ, dArgs, dMap, shape.blocks, stream);
+    launch_xy();
+  }
+}
+
+template <typename T>
+int ExecutionGPU<T>::tuned_plane_choice(int n, bool forward) {
+  const int forced = plane_kernels_forced();
+  if (forced >= 0) return forced;
+  TuneTable& tt = tune_table();
+  std::lock_guard<std::mutex> lock(tt.mutex);
+  auto it = tt.choice.find(std::make_tuple(static_cast<int>(sizeof(T)), n, forward ? 1 : 0));
+  return it == tt.choice.end() ? -1 : it->second;
+}
+
+template <typename T>
+bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
+                                const std::vector<const T*>& in, const std::vector<T*>& out,
+                                bool forward, const std::vector<SpfftScalingType>& scaling) {
+  const size_t n = ex.size();
+  std::vector<const T*> din(n);
+  std::vector<T*> dout(n);
+  BatchShape shape;
+  if (!batch_qualifies(ex.data(), n, in.data(), out.data(), din.data(), dout.data(), &shape)) {
+    return false;
+  }
+  const int nt = static_cast<int>(n);
 
   std::vector<std::uint64_t> key;
   key.reserve(3 + 4 * n);
@@ -490,39 +832,8 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
     key.push_back(forward && scaling[t] == SPFFT_FULL_SCALING ? 1 : 0);
   }
 
-  int S = 0, LZ = 0, shift = 0;
-  fast_z_geometry<T>(m0.dimZ, &S, &LZ, &shift);
-  int blocks = 0, maxNcol = 0;
-  for (size_t t = 0; t < n; ++t) {
-    blocks += (ex[t]->map_->num_sticks() + S - 1) / S;
-    maxNcol = std::max(maxNcol, ex[t]->numCols_);
-  }
-  const bool r2c = m0.type == SPFFT_TRANS_R2C;
-  const int nt = static_cast<int>(n);
-  // whole-plane kernels are a candidate when every transform has its tile
-  // positions and the widest tile fits
-  bool planeOk = plane_kernels_forced() != 0 && plane_path_supported<T>(m0.dimX, maxNcol, r2c);
-  for (size_t t = 0; t < n && planeOk; ++t) planeOk = ex[t]->pitchT_ > 0;
+  const bool planeOk = plane_candidate(ex.data(), n, din.data(), dout.data(), forward, shape);
   bool plane = false;
-  auto launch_xy_variant = [&](bool usePlane, const FastArgs<T>* dArgs, hipStream_t s) {
-    if (usePlane) {
-      launch_plane_xy_batch<T>(forward, m0.dimX, dArgs, nt, m0.dimZ, maxNcol, s);
-    } else {
-      launch_fast_xy_batch<T>(forward, r2c, m0.dimX, dArgs, nt, m0.dimZ, maxNcol, s);
-    }
-  };
-  auto launch_xy = [&](const FastArgs<T>* dArgs, hipStream_t s) {
-    launch_xy_variant(plane, dArgs, s);
-  };
-  auto launch_chain = [&](const FastArgs<T>* dArgs, const int2* dMap, hipStream_t s) {
-    if (forward) {
-      launch_xy(dArgs, s);
-      launch_fast_z_batch<T>(true, r2c, m0.dimZ, dArgs, dMap, blocks, s);
-    } else {
-      launch_fast_z_batch<T>(false, r2c, m0.dimZ, dArgs, dMap, blocks, s);
-      launch_xy(dArgs, s);
-    }
-  };
 
   BatchCache& cache = batch_cache();
   std::lock_guard<std::mutex> lock(cache.mutex);
@@ -536,18 +847,9 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
       cache.plans.pop_back();
     }
     std::vector<FastArgs<T>> args(n);
-    std::vector<int2> zmap;
-    zmap.reserve(blocks);
-    for (size_t t = 0; t < n; ++t) {
-      FastArgs<T> f = ex[t]->fast_args();
-      f.in = din[t];
-      f.out = dout[t];
-      f.useScale = static_cast<int>(key[3 + 4 * t + 3]);
-      f.pitchA = fast_xy_pitch_a<T>(m0.dimX, f.ncol);
-      const int b = (f.nst + S - 1) / S;
-      for (int k = 0; k < b; ++k) zmap.push_back(make_int2(static_cast<int>(t), k));
-      args[t] = f;
-    }
+    std::vector<int2> zmap(shape.blocks);
+    fill_batch_tables(ex.data(), n, din.data(), dout.data(), forward,
+                      forward ? scaling.data() : nullptr, args.data(), zmap.data());
     BatchPlan plan;
     plan.key = std::move(key);
     check_hip(hipMalloc(&plan.args, n * sizeof(FastArgs<T>)));
@@ -567,29 +869,34 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
   const auto* dMap = static_cast<const int2*>(plan.map);
 
   if (planeOk) {
-    const int forced = plane_kernels_forced();
-    const auto tkey = std::make_tuple(static_cast<int>(sizeof(T)), m0.dimX, forward ? 1 : 0);
-    TuneTable& tt = tune_table();
-    std::lock_guard<std::mutex> tlock(tt.mutex);
-    auto it = tt.choice.find(tkey);
-    if (forced == 1) {
-      plane = true;
-    } else if (it != tt.choice.end()) {
-      plane = it->second != 0;
+    const int choice = tuned_plane_choice(shape.dimX, forward);
+    if (choice >= 0) {
+      plane = choice != 0;
     } else {
       // Time both plane kernels on this batch. The xy launch is idempotent
       // (backward: [z][stick] -> outputs, after the z launch has filled the
       // buffer; forward: inputs -> [z][stick]), and the call's real launch
       // chain follows.
-      if (!forward) launch_fast_z_batch<T>(false, r2c, m0.dimZ, dArgs, dMap, blocks, nullptr);
+      if (!forward) {
+        launch_fast_z_batch<T>(false, shape.r2c, shape.dimZ, dArgs, dMap, shape.blocks, nullptr);
+      }
       hipEvent_t e0 = nullptr, e1 = nullptr;
       check_hip(hipEventCreate(&e0));
       check_hip(hipEventCreate(&e1));
       float ms[2] = {0.f, 0.f};
       for (int c = 0; c < 2; ++c) {
-        launch_xy_variant(c == 1, dArgs, nullptr);
+        auto xy = [&] {
+          if (c == 1) {
+            launch_plane_xy_batch<T>(forward, shape.dimX, dArgs, nt, shape.dimZ, shape.maxNcol,
+                                     nullptr);
+          } else {
+            launch_fast_xy_batch<T>(forward, shape.r2c, shape.dimX, dArgs, nt, shape.dimZ,
+                                    shape.maxNcol, nullptr);
+          }
+        };
+        xy();
         check_hip(hipEventRecord(e0, nullptr));
-        for (int k = 0; k < 3; ++k) launch_xy_variant(c == 1, dArgs, nullptr);
+        for (int k = 0; k < 3; ++k) xy();
         check_hip(hipEventRecord(e1, nullptr));
         check_hip(hipEventSynchronize(e1));
         check_hip(hipEventElapsedTime(&ms[c], e0, e1));
@@ -597,11 +904,16 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
       plane = ms[1] < ms[0];
-      tt.choice[tkey] = plane ? 1 : 0;
+      {
+        TuneTable& tt = tune_table();
+        std::lock_guard<std::mutex> tlock(tt.mutex);
+        tt.choice[std::make_tuple(static_cast<int>(sizeof(T)), shape.dimX, forward ? 1 : 0)] =
+            plane ? 1 : 0;
+      }
       if (tune_log()) {
         std::fprintf(stderr, "spfft_amd tune: %s N=%d %s, %zu transforms: class-split %.3f ms, "
                      "whole-plane %.3f ms -> %s\n", sizeof(T) == 8 ? "double" : "float",
-                     m0.dimX, forward ? "forward" : "backward", n, ms[0] / 3, ms[1] / 3,
+                     shape.dimX, forward ? "forward" : "backward", n, ms[0] / 3, ms[1] / 3,
                      plane ? "whole-plane" : "class-split");
       }
     }
@@ -613,7 +925,7 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
     plan.graph = nullptr;
   }
   if (!graphs_enabled() || prof::enabled()) {
-    launch_chain(dArgs, dMap, nullptr);
+    launch_batch(forward, shape, nt, plane, dArgs, dMap, nullptr);
     return true;
   }
   if (!plan.exec) {
@@ -624,7 +936,7 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
     check_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
     try {
-      launch_chain(dArgs, dMap, cs);
+      launch_batch(forward, shape, nt, plane, dArgs, dMap, cs);
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(cs, &g);

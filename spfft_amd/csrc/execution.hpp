@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <array>
 #include <cstdint>
 #include <memory>
 #include <vector>
@@ -17,6 +18,26 @@
 #include "spfft/types.h"
 
 namespace spfft {
+
+// Device index tables and twiddles of one index set on one device. Built once
+// and shared by every transform with an identical set (same dimensions, type,
+// precision and value order), so the bands of one set hold, and a batch of
+// them reads, a single copy. Immutable after construction.
+template <typename T>
+struct TransformTables {
+  std::shared_ptr<const IndexMap> map;
+  int device = 0;
+  bool allowFast = true;  // built with the fixed-length path allowed
+  bool fast = false;      // fixed-length kernels serve this set
+  int numCols = 0;
+  int pitchT = 0;  // whole-plane batch kernels' tile pitch (0: not applicable)
+  DeviceBuffer<Cx<T>> twX, twY, twZ;
+  DeviceBuffer<int> entryValue, entrySlot, stickBegin, stickY, stickCol, colX, colBegin;
+  DeviceBuffer<int> stickAt;
+  DeviceBuffer<int2> zEntries;
+  DeviceBuffer<int4> zInfo;  // stick runs, when every stick is one run in user order
+  DeviceBuffer<int> resSticks, resBegin, xCol, resYM;
+};
 
 template <typename T>
 class ExecutionGPU {
@@ -50,15 +71,62 @@ public:
                         const std::vector<T*>& out, bool forward,
                         const std::vector<SpfftScalingType>& scaling);
 
+  // Pieces of the fused batch, shared by run_batch and the allocation-free
+  // batched entry point (which keeps its tables in a static device array).
+  struct BatchShape {
+    int blocks = 0;   // z workgroups of the whole batch
+    int maxNcol = 0;  // widest column set
+    bool r2c = false;
+    int dimX = 0, dimZ = 0;
+  };
+  // True when the n transforms can run as one fused chain; then din/dout (n
+  // entries each) hold the device views of the user pointers.
+  static bool batch_qualifies(ExecutionGPU* const* ex, size_t n, const T* const* in,
+                              T* const* out, const T** din, T** dout, BatchShape* shape);
+  // True when the whole-plane kernels can serve the batch.
+  static bool plane_candidate(ExecutionGPU* const* ex, size_t n, const T* const* din,
+                              T* const* dout, bool forward, const BatchShape& shape);
+  // Host copies of the argument table (n entries) and the z block map
+  // (shape.blocks entries); `scaling` is read for forward batches only.
+  static void fill_batch_tables(ExecutionGPU* const* ex, size_t n, const T* const* din,
+                                T* const* dout, bool forward, const SpfftScalingType* scaling,
+                                FastArgs<T>* args, int2* zmap);
+  static void launch_batch(bool forward, const BatchShape& shape, int n, bool plane,
+                           const FastArgs<T>* dArgs, const int2* dMap, hipStream_t stream);
+  // The plane-kernel family chosen for (precision, length, direction): 1
+  // whole-plane, 0 class-split, -1 not decided yet.
+  static int tuned_plane_choice(int n, bool forward);
+
   T* space_domain_gpu() const { return grid_->space_gpu(); }
   T* space_domain_host() const { return grid_->space_host(); }
   hipStream_t stream() const { return stream_; }
   // Number of T values in the space domain (2 per element for C2C).
   size_t space_domain_size() const;
 
+  // True for device or managed memory (used in place, no staging).
+  static bool device_accessible(const void* ptr);
+
+  // Chunks of a host transfer (copies overlap kernels and host copies).
+  static constexpr int kIoMaxChunks = 64;
+
 private:
   // Checks for preceding errors and returns the stream to enqueue on.
   hipStream_t begin(bool concurrent);
+  // Host I/O: events and the grid's two-slot pinned ring of slotBytes each.
+  void io_prepare(size_t slotBytes);
+  int planes_per_chunk(size_t planeBytes) const;
+  // Host -> device piece k through the ring (pageable) or directly (pinned),
+  // on `copy`; event k is recorded after it.
+  void upload_piece(char* dst, const char* src, size_t bytes, bool pinned, int k,
+                    hipStream_t copy);
+  // A whole host range in ring-sized pieces; `compute` waits for the last.
+  void upload(char* dst, const char* src, size_t bytes, bool pinned, hipStream_t compute,
+              hipStream_t copy);
+  // Device -> host in pieces of pieceBytes; piece k waits for ready[min(k,
+  // readyCount - 1)] (recorded on `compute`). Pageable destinations are filled
+  // before the call returns; pinned ones when `compute` reaches its end.
+  void download(char* dst, const char* src, size_t bytes, size_t pieceBytes, bool pinned,
+                const hipEvent_t* ready, int readyCount, hipStream_t compute, hipStream_t copy);
   StageArgs<T> base_args() const;
   FastArgs<T> fast_args() const;
 
@@ -71,16 +139,16 @@ private:
   int numCols_ = 0;
   int zBatch_ = 1, yBatch_ = 1, xBatch_ = 1;
   FftDesc fftX_, fftY_, fftZ_;
-  DeviceBuffer<Cx<T>> twX_, twY_, twZ_;
-  DeviceBuffer<int> entryValue_, entrySlot_, stickBegin_, stickY_, stickCol_, colX_, colBegin_;
-  // fixed-length C2C path (two kernels per direction)
+  std::shared_ptr<TransformTables<T>> tab_;  // shared with transforms of the same set
+  // fixed-length path (two kernels per direction)
   bool fast_ = false;
-  DeviceBuffer<int> stickAt_;
-  DeviceBuffer<int2> zEntries_;
-  DeviceBuffer<int4> zInfo_;  // stick runs, when every stick is one run in user order
-  DeviceBuffer<int> resSticks_, resBegin_, xCol_, resYM_;
   // whole-plane batch kernels (power-of-two lengths up to 128, C2C): tile pitch
   int pitchT_ = 0;
+  // host I/O (created on the first call with a host pointer): chunk-ready
+  // events [0, kIoMaxChunks), copy-done events [kIoMaxChunks, 2 kIoMaxChunks)
+  std::array<hipEvent_t, 2 * kIoMaxChunks> ioEvents_{};
+  char* ring_ = nullptr;
+  size_t slotBytes_ = 0;
 };
 
 }  // namespace spfft

@@ -335,7 +335,7 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
   Cx<T>* twl = tile + Z::TW;
   const int s0 = bid * S;
   const int ns = min(S, a.nst - s0);
-  if (a.zInfo) {
+  if (a.hasRuns) {
     z_backward_runs<T, NZ, RC, STREAM>(a, bid);
     return;
   }
@@ -412,7 +412,7 @@ __device__ __forceinline__ void z_forward_body(const FastArgs<T>& a, int bid) {
                                         : Cx<T>{T(0), T(0)};
   }
   if (threadIdx.x < NZ) twl[threadIdx.x] = a.twZ[threadIdx.x];
-  if (a.zInfo) {
+  if (a.hasRuns) {
     // sticks stored as runs: each lane writes its outputs straight to the
     // run (see z_backward_runs)
     const int lane = threadIdx.x % kWave;
@@ -529,6 +529,7 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
   Cx<T>* A = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
   int z, r;
   plane_class<R>(bid, z, r);
+  z += a.zBegin;
   if (z >= a.nz) return;
   const Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -691,6 +692,7 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
   Cx<T>* Tb = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
   int z, c;
   plane_class<R>(bid, z, c);
+  z += a.zBegin;
   if (z >= a.nz) return;
   const Cx<T>* in = reinterpret_cast<const Cx<T>*>(a.in) + static_cast<size_t>(z) * N * N;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -1047,8 +1049,10 @@ __device__ __forceinline__ void full_rows(const FastArgs<T>& a, int z, int line,
 // would access them with flat instructions (which share the LDS counter and
 // serialise against LDS traffic). Rebasing each pointer on a kernel argument
 // (a global pointer) keeps the address and lets the accesses be global ones.
+// A null pointer stays null (no offset is formed from it).
 template <typename P>
 __device__ __forceinline__ P* rebase(const char* anchor, P* p) {
+  if (p == nullptr) return nullptr;
   return (P*)(anchor + (reinterpret_cast<const char*>(p) - anchor));
 }
 template <typename T>
@@ -1413,7 +1417,7 @@ void xy_backward_n(FastArgs<T> a, hipStream_t stream) {
   auto kernel = a.r2c ? k_xy_backward_fast<T, N, true> : k_xy_backward_fast<T, N, false>;
   prepare_lds(kernel, lds);
   prof::KernelSpan span(stream, a.r2c ? "k_xy_backward_r2c" : "k_xy_backward_fast", true);
-  hipExtLaunchKernelGGL(kernel, dim3(plane_grid<N>(a.nz)), dim3(S::NT), lds, stream,
+  hipExtLaunchKernelGGL(kernel, dim3(plane_grid<N>(a.nz - a.zBegin)), dim3(S::NT), lds, stream,
                         span.start(), span.stop(), 0, a);
   check_launch();
 }
@@ -1425,7 +1429,7 @@ void xy_forward_n(const FastArgs<T>& a, hipStream_t stream) {
   auto kernel = a.r2c ? k_xy_forward_fast<T, N, true> : k_xy_forward_fast<T, N, false>;
   prepare_lds(kernel, lds);
   prof::KernelSpan span(stream, a.r2c ? "k_xy_forward_r2c" : "k_xy_forward_fast", true);
-  hipExtLaunchKernelGGL(kernel, dim3(plane_grid<N>(a.nz)), dim3(S::NT), lds, stream,
+  hipExtLaunchKernelGGL(kernel, dim3(plane_grid<N>(a.nz - a.zBegin)), dim3(S::NT), lds, stream,
                         span.start(), span.stop(), 0, a);
   check_launch();
 }

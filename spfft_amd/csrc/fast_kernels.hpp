@@ -31,6 +31,7 @@ namespace spfft {
 template <typename T>
 struct FastArgs {
   int nz = 0, nst = 0, ncol = 0;
+  int zBegin = 0;  // single-transform xy launches: planes [zBegin, nz) (chunked host I/O)
   int pitch = 0;   // row pitch of zbuf in elements (nst + 1 rounded up to 128 bytes);
                    // element nst of every row is zero
   int pitchA = 0;  // row pitch of the backward xy kernel's column tile (set at launch)
@@ -52,6 +53,8 @@ struct FastArgs {
                                       //  LDS tile position | repeated << 31}
   const int4* zInfo = nullptr;        // per stick {first value, first storage z, length} when
                                       // every stick's values are one run in user order
+  int hasRuns = 0;                    // zInfo is valid (set on the host; kernels select the
+                                      // stick-run path on this flag, never on the pointer)
   const int* xCol = nullptr;          // [NX]: column id of x, or ncol (a zero column)
   const int* resSticks = nullptr;     // sticks grouped by colX mod R
   const int* resBegin = nullptr;      // R + 1 offsets into resSticks

@@ -49,12 +49,20 @@ GridInternal<T>::GridInternal(const GridInternal& other)
   if (processingUnit_ & SPFFT_PU_GPU) {
     DeviceGuard guard(deviceId_);
     allocate();
+  } else {
+    allocate();
   }
 }
 
 template <typename T>
 void GridInternal<T>::allocate() {
-  if (!(processingUnit_ & SPFFT_PU_GPU)) return;
+  const size_t hostSize = 2 * static_cast<size_t>(maxDimX_) * maxDimY_ * maxDimZ_;
+  if (!(processingUnit_ & SPFFT_PU_GPU)) {
+    // host-only grid: no HIP call at all
+    host1_.reset(new HostBuffer<T>(hostSize, false));
+    host2_.reset(new HostBuffer<T>(hostSize, false));
+    return;
+  }
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
     (void)hipGetLastError();
@@ -69,16 +77,31 @@ void GridInternal<T>::allocate() {
   // sticks)
   const size_t pitch = (static_cast<size_t>(maxNumLocalZColumns_) + 1 + 15) / 16 * 16;
   sticks_ = DeviceBuffer<T>(2 * static_cast<size_t>(maxDimZ_) * pitch);
+  if (processingUnit_ & SPFFT_PU_HOST) {
+    host1_.reset(new HostBuffer<T>(hostSize, true));
+    host2_.reset(new HostBuffer<T>(hostSize, true));
+  }
 }
 
 template <typename T>
 T* GridInternal<T>::space_host() {
+  if (host1_) return host1_->data();
   std::lock_guard<std::mutex> lock(hostMutex_);
   if (!host_) {
     const size_t gridSize = static_cast<size_t>(maxDimX_) * maxDimY_ * maxDimZ_;
     host_.reset(new PinnedBuffer<T>(2 * gridSize));
   }
   return host_->data();
+}
+
+template <typename T>
+void* GridInternal<T>::host_staging(size_t bytes) {
+  std::lock_guard<std::mutex> lock(hostMutex_);
+  if (!staging_ || staging_->size() < bytes) {
+    staging_.reset();
+    staging_.reset(new HostBuffer<char>(bytes, true));
+  }
+  return staging_->data();
 }
 
 template class GridInternal<double>;

@@ -6,7 +6,12 @@
 //   space : X*Y*Z complex  - space-domain data (real for R2C uses the front half)
 //   work  : X*Y*Z complex  - y-stage intermediate [z][y][column] and host staging
 //   sticks: Z*C' complex   - z-stage intermediate [z][stick], C' = C rounded up to 16
-// The pinned host mirror of the space domain is allocated on first use.
+// Host memory, for grids whose processing unit includes SPFFT_PU_HOST (pinned
+// when it also includes SPFFT_PU_GPU, as in reference grid_internal.cpp:85-91):
+//   array 1: X*Y*Z complex - host execution's sticks and space domain
+//   array 2: X*Y*Z complex - host execution's planes
+// Array 1 is also the host mirror of the space domain for GPU transforms; a
+// GPU-only grid allocates a pinned mirror on first use instead.
 #pragma once
 
 #include <memory>
@@ -38,8 +43,15 @@ public:
   T* space_gpu() const { return space_.data(); }
   T* work_gpu() const { return work_.data(); }
   T* sticks_gpu() const { return sticks_.data(); }
-  // Pinned host buffer of X*Y*Z complex values, allocated on first call.
+  // Host space-domain mirror of X*Y*Z complex values: array 1, or for a
+  // GPU-only grid a pinned buffer allocated on first call.
   T* space_host();
+  // Pinned staging for host transfers of GPU transforms, at least `bytes`
+  // (allocated on first use, grown when a transform needs more).
+  void* host_staging(size_t bytes);
+  // Host execution buffers (null unless the grid includes SPFFT_PU_HOST).
+  T* host_array1() const { return host1_ ? host1_->data() : nullptr; }
+  T* host_array2() const { return host2_ ? host2_->data() : nullptr; }
 
 private:
   void allocate();
@@ -50,6 +62,8 @@ private:
   int deviceId_ = 0;
   DeviceBuffer<T> space_, work_, sticks_;
   std::unique_ptr<PinnedBuffer<T>> host_;
+  std::unique_ptr<HostBuffer<T>> host1_, host2_;
+  std::unique_ptr<HostBuffer<char>> staging_;
   std::mutex hostMutex_;
 };
 

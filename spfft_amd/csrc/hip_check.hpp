@@ -23,6 +23,13 @@ inline void check_hip(hipError_t status) {
   }
 }
 
+// An error left by earlier GPU work is reported, not absorbed (reference
+// execution_gpu.cpp:256-258). Called first in every GPU entry point, before
+// the call's own HIP calls (a pointer query may clear the thread's error).
+inline void check_preceding() {
+  if (hipGetLastError() != hipSuccess) throw GPUPrecedingError();
+}
+
 // Check the launch that was just enqueued.
 inline void check_launch() {
   const hipError_t status = hipGetLastError();

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstddef>
+#include <new>
 #include <utility>
 #include <vector>
 
@@ -90,6 +91,46 @@ public:
 private:
   T* ptr_ = nullptr;
   size_t count_ = 0;
+};
+
+// Host array: pinned (hipHostMalloc) or ordinary 64-byte aligned memory.
+template <typename T>
+class HostBuffer {
+public:
+  HostBuffer(size_t count, bool pinned) : count_(count), pinned_(pinned) {
+    if (count_ == 0) return;
+    if (pinned_) {
+      const hipError_t status = hipHostMalloc(reinterpret_cast<void**>(&ptr_),
+                                              count_ * sizeof(T), hipHostMallocDefault);
+      if (status != hipSuccess) {
+        (void)hipGetLastError();
+        ptr_ = nullptr;
+        throw HostAllocationError();
+      }
+    } else {
+      ptr_ = static_cast<T*>(::operator new(count_ * sizeof(T), std::align_val_t(64),
+                                            std::nothrow));
+      if (!ptr_) throw HostAllocationError();
+    }
+  }
+  ~HostBuffer() {
+    if (!ptr_) return;
+    if (pinned_) {
+      (void)hipHostFree(ptr_);
+    } else {
+      ::operator delete(ptr_, std::align_val_t(64));
+    }
+  }
+  HostBuffer(const HostBuffer&) = delete;
+  HostBuffer& operator=(const HostBuffer&) = delete;
+
+  T* data() const { return ptr_; }
+  size_t size() const { return count_; }
+
+private:
+  T* ptr_ = nullptr;
+  size_t count_ = 0;
+  bool pinned_ = false;
 };
 
 }  // namespace spfft

@@ -24,9 +24,10 @@ TransformInternal<T>::TransformInternal(SpfftProcessingUnitType processingUnit,
   }
   if (map_->dimX <= 0 || map_->dimY <= 0 || map_->dimZ <= 0) throw InvalidParameterError();
   if (processingUnit == SPFFT_PU_HOST) {
-    // This library executes on the GPU only; host execution is the
-    // reference's FFTW path and is not provided here.
-    throw HostExecutionError();
+    // CPU execution on the grid's host arrays (reference
+    // transform_internal.cpp:90-97); no HIP call is made for it
+    execHost_.reset(new ExecutionHost<T>(map_, grid_, grid_->num_threads()));
+    return;
   }
   DeviceGuard guard(grid_->device_id());
   exec_.reset(new ExecutionGPU<T>(map_, grid_));
@@ -42,6 +43,12 @@ TransformInternal<T> TransformInternal<T>::clone() const {
 
 template <typename T>
 T* TransformInternal<T>::space_domain_data(SpfftProcessingUnitType location) {
+  if (execHost_) {
+    // a host transform has its space domain in host memory only
+    // (reference transform_internal.cpp:341-356)
+    if (location != SPFFT_PU_HOST) throw InvalidParameterError();
+    return execHost_->space_domain();
+  }
   return location == SPFFT_PU_GPU ? exec_->space_domain_gpu() : exec_->space_domain_host();
 }
 
@@ -53,6 +60,11 @@ void TransformInternal<T>::forward(SpfftProcessingUnitType inputLocation, T* out
 
 template <typename T>
 void TransformInternal<T>::forward(const T* input, T* output, SpfftScalingType scaling) {
+  if (execHost_) {
+    execHost_->forward(input, output, scaling);
+    return;
+  }
+  check_preceding();
   DeviceGuard guard(grid_->device_id());
   exec_->forward(input, output, scaling);
   exec_->synchronize(execMode_);
@@ -65,6 +77,11 @@ void TransformInternal<T>::backward(const T* input, SpfftProcessingUnitType outp
 
 template <typename T>
 void TransformInternal<T>::backward(const T* input, T* output) {
+  if (execHost_) {
+    execHost_->backward(input, output);
+    return;
+  }
+  check_preceding();
   DeviceGuard guard(grid_->device_id());
   exec_->backward(input, output);
   exec_->synchronize(execMode_);
@@ -72,18 +89,29 @@ void TransformInternal<T>::backward(const T* input, T* output) {
 
 template <typename T>
 void TransformInternal<T>::forward_enqueue(const T* input, T* output, SpfftScalingType scaling) {
+  if (execHost_) {
+    execHost_->forward(input, output, scaling);
+    return;
+  }
+  check_preceding();
   DeviceGuard guard(grid_->device_id());
   exec_->forward(input, output, scaling, true);
 }
 
 template <typename T>
 void TransformInternal<T>::backward_enqueue(const T* input, T* output) {
+  if (execHost_) {
+    execHost_->backward(input, output);
+    return;
+  }
+  check_preceding();
   DeviceGuard guard(grid_->device_id());
   exec_->backward(input, output, true);
 }
 
 template <typename T>
 void TransformInternal<T>::synchronize() {
+  if (execHost_) return;  // host execution is synchronous
   DeviceGuard guard(grid_->device_id());
   exec_->synchronize(execMode_, true);
 }
@@ -110,6 +138,10 @@ bool TransformInternal<T>::batch_backward(const std::vector<TransformInternal*>&
                                           const std::vector<const T*>& in,
                                           const std::vector<T*>& out) {
   if (ts.empty()) return false;
+  for (TransformInternal* t : ts) {
+    if (!t->exec_) return false;  // host transforms run one by one
+  }
+  check_preceding();
   std::vector<std::shared_ptr<ExecutionGPU<T>>> execs;
   for (TransformInternal* t : ts) execs.push_back(t->exec_);
   return run_batched<T>(ts, execs, in, out, false, {}, ts[0]->device_id());
@@ -121,6 +153,10 @@ bool TransformInternal<T>::batch_forward(const std::vector<TransformInternal*>& 
                                          const std::vector<T*>& out,
                                          const std::vector<SpfftScalingType>& scaling) {
   if (ts.empty()) return false;
+  for (TransformInternal* t : ts) {
+    if (!t->exec_) return false;  // host transforms run one by one
+  }
+  check_preceding();
   std::vector<std::shared_ptr<ExecutionGPU<T>>> execs;
   for (TransformInternal* t : ts) execs.push_back(t->exec_);
   return run_batched<T>(ts, execs, in, out, true, scaling, ts[0]->device_id());

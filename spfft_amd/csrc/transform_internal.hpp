@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "execution.hpp"
+#include "execution_host.hpp"
 #include "grid_internal.hpp"
 #include "index_map.hpp"
 #include "spfft/types.h"
@@ -66,7 +67,8 @@ private:
   SpfftExecType execMode_ = SPFFT_EXEC_SYNCHRONOUS;
   std::shared_ptr<IndexMap> map_;
   std::shared_ptr<GridInternal<T>> grid_;
-  std::shared_ptr<ExecutionGPU<T>> exec_;
+  std::shared_ptr<ExecutionGPU<T>> exec_;       // SPFFT_PU_GPU transforms
+  std::shared_ptr<ExecutionHost<T>> execHost_;  // SPFFT_PU_HOST transforms
 };
 
 }  // namespace spfft

@@ -148,3 +148,69 @@ def test_examples_compile():
                    check=True)
     subprocess.run(["g++", "-std=c++17", "-Wall", "-I", INCLUDE, os.path.join(ex, "example.cpp"),
                     "-o", os.path.join(out, "example_cpp"), "-L", libdir, "-lspfft"], check=True)
+
+
+def _null_handle_args(name, argtypes):
+    """Arguments for `name` with a null handle first and valid-looking
+    storage for every other parameter."""
+    keep = []
+    args = [None]
+    for t in argtypes[1:]:
+        if t is ctypes.c_int:
+            args.append(0)
+        elif t is ctypes.c_void_p:
+            args.append(None)
+        else:  # pointer parameters: room for one value
+            buf = (ctypes.c_longlong * 4)()
+            keep.append(buf)
+            args.append(ctypes.cast(buf, t))
+    return args, keep
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_every_entry_point_rejects_a_null_handle(precision):
+    """All 36 functions per precision (reference transform.cpp:178-562,
+    grid.cpp:88-290, multi_transform.cpp:84-147): a null grid or transform
+    handle gives SPFFT_INVALID_HANDLE_ERROR (the create functions take an
+    output handle and a grid instead). A null transform inside a
+    multi_transform array is reported as SPFFT_INVALID_PARAMETER_ERROR (the
+    reference dereferences it); an empty batch succeeds."""
+    api = sp._API[precision]
+    assert len(api) == 36
+    seen = 0
+    for name, f in api.items():
+        if name.startswith("multi_transform"):
+            n = 1
+            handles = (ctypes.c_void_p * 1)(None)
+            rest = [(ctypes.c_void_p * 1)(None) if t is not ctypes.POINTER(ctypes.c_int)
+                    else (ctypes.c_int * 1)(0) for t in f.argtypes[2:]]
+            assert f(n, handles, *rest) == sp.SPFFT_INVALID_PARAMETER_ERROR, name
+            assert f(0, handles, *rest) == sp.SPFFT_SUCCESS, name
+        elif name == "grid_create":
+            continue
+        elif name == "transform_create_independent":
+            continue
+        elif name == "transform_create":
+            out = ctypes.c_void_p()
+            idx = (ctypes.c_int * 3)(0, 0, 0)
+            assert f(ctypes.byref(out), None, sp.SPFFT_PU_HOST, 0, 4, 4, 4, 4, 1, 0, idx) == \
+                sp.SPFFT_INVALID_HANDLE_ERROR
+        else:
+            args, keep = _null_handle_args(name, f.argtypes)
+            assert f(*args) == sp.SPFFT_INVALID_HANDLE_ERROR, name
+        seen += 1
+    assert seen == 34
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_grid_create_argument_errors(precision):
+    """Grid dimension and processing-unit checks run before any allocation
+    (reference grid_internal.cpp:49-67)."""
+    api = sp._API[precision]
+    h = ctypes.c_void_p()
+    bad = [(0, 4, 4, 4, sp.SPFFT_PU_GPU), (4, -1, 4, 4, sp.SPFFT_PU_GPU),
+           (4, 4, 0, 4, sp.SPFFT_PU_HOST), (4, 4, 4, -1, sp.SPFFT_PU_HOST),
+           (4, 4, 4, 4, 0), (4, 4, 4, 4, 4)]
+    for dims in bad:
+        assert api["grid_create"](ctypes.byref(h), *dims, -1) == \
+            sp.SPFFT_INVALID_PARAMETER_ERROR, dims

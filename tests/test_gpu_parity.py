@@ -279,11 +279,11 @@ FAST_LENGTHS = [32, 64, 96, 128, 192]
                          ids=["dense70", "sparse_centred", "full"])
 def test_fixed_length_path_random_sets(n, fill, precision, r2c):
     """Random index sets on the fixed-length kernels (every column pattern,
-    including fully dense planes). For R2C the values are arbitrary on the
-    half space, so the stick and plane symmetry fills decide the result."""
+    including fully dense planes; a dense 192^3 C2C set has more columns than
+    the fixed-length LDS tile holds and runs on the generic kernels). For R2C
+    the values are arbitrary on the half space, so the stick and plane
+    symmetry fills decide the result."""
     sf, ff, centered = fill
-    if n == 192 and sf == 1.0 and not r2c:
-        pytest.skip("dense 192^3 column set exceeds the fixed-length LDS budget (generic path)")
     dims = (n, n, n)
     t = random_index_set(70 + n, dims, r2c, stick_fraction=sf, fill_fraction=ff,
                          centered=centered)

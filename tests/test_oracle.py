@@ -273,3 +273,24 @@ def test_oracle_matches_golden_fixtures(path):
         assert abs(np.linalg.norm(b) - float(g["backward_norm"])) <= 1e-12 * float(g["backward_norm"])
     space = b.reshape(dims[2], dims[1], dims[0])
     assert orc.rel_l2(o.forward(space, True), g["forward_scaled"]) <= 1e-13
+
+
+@pytest.mark.parametrize("dims,r2c", [((11, 12, 13), False), ((12, 11, 13), True),
+                                      ((32, 32, 32), False), ((16, 1, 9), True)])
+def test_fftw_api_provider_matches_builtin(dims, r2c):
+    """The CPU baseline may route the oracle's 1D lines through a library
+    exporting the FFTW3 C API (FFTW or MKL): same results as the built-in FFT
+    to rounding, backward and forward."""
+    t = random_index_set(11, dims, r2c, centered=True)
+    v = complex_values(12, len(t))
+    orc.use_builtin_fft()
+    p0 = orc.OraclePlan(r2c, dims, t)
+    if orc.use_fft_library() is None:
+        pytest.skip("no library with the FFTW3 C API on this host")
+    try:
+        p1 = orc.OraclePlan(r2c, dims, t)
+        w0, w1 = p0.backward(v), p1.backward(v)
+        assert orc.rel_l2(w1, w0) <= 1e-14
+        assert orc.rel_l2(p1.forward(w0, True), p0.forward(w0, True)) <= 1e-14
+    finally:
+        orc.use_builtin_fft()
