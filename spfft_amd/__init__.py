@@ -321,6 +321,24 @@ def multi_transform_forward(transforms, inputs, outputs, scalings):
                ("int", scalings))
 
 
+lib.spfft_amd_multi_transform_noalloc.argtypes = [_i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                  ctypes.POINTER(_vp), _i, _ip]
+lib.spfft_amd_multi_transform_noalloc.restype = ctypes.c_int
+
+
+def multi_transform_noalloc(transforms, inputs, outputs, forward, scalings=None):
+    """Allocation-free batched call (include/spfft_amd/no_alloc.h): double
+    GPU transforms of one shape, device pointers only; `scalings` is read for
+    forward calls."""
+    n = len(transforms)
+    hs = (_vp * max(n, 1))(*[t.handle for t in transforms])
+    ins = (_vp * max(n, 1))(*[_ptr(a) for a in inputs])
+    outs = (_vp * max(n, 1))(*[_ptr(a) for a in outputs])
+    sc = (_i * max(n, 1))(*[int(a) for a in (scalings or [SPFFT_NO_SCALING] * n)])
+    _check(lib.spfft_amd_multi_transform_noalloc(n, hs, ins, outs, 1 if forward else 0, sc),
+           "spfft_amd_multi_transform_noalloc")
+
+
 def multi_transform_backward(transforms, inputs, outputs):
     """outputs: list of pointers, or list of processing-unit locations (ints)."""
     prec = transforms[0]._precision if transforms else "double"

@@ -15,7 +15,9 @@
 
 #include "grid_internal.hpp"
 #include "index_map.hpp"
+#include "no_alloc.hpp"
 #include "spfft/exceptions.hpp"
+#include "spfft_amd/no_alloc.h"
 #include "spfft/spfft.h"
 #include "spfft/spfft.hpp"
 #include "transform_internal.hpp"
@@ -121,6 +123,25 @@ public:
       outputs[t] = transforms[t].transform_->space_domain_data(locations[t]);
     }
     backward(n, transforms, inputs, outputs.data());
+  }
+
+  // Allocation-free batched call (no counterpart in the reference): the
+  // C handles are used in place, no C++ objects are built.
+  static void noalloc(int n, void* const* handles, const ValueType* const* inputs,
+                      ValueType* const* outputs, bool forward, const SpfftScalingType* scalings) {
+    if (n < 1 || n > kNoAllocMaxTransforms || !handles) throw InvalidParameterError();
+    TransformInternal<ValueType>* ts[kNoAllocMaxTransforms];
+    for (int t = 0; t < n; ++t) {
+      if (!handles[t]) throw InvalidParameterError();
+      ts[t] = reinterpret_cast<TransformType*>(handles[t])->transform_.get();
+    }
+    for (int a = 0; a < n; ++a) {
+      for (int b = a + 1; b < n; ++b) {
+        if (ts[a]->shared_grid(*ts[b])) throw InvalidParameterError();
+      }
+    }
+    TransformInternal<ValueType>::batch_noalloc(ts, static_cast<size_t>(n), inputs, outputs,
+                                                forward, scalings);
   }
 
 private:
@@ -484,6 +505,21 @@ std::vector<TransformType> gather(int n, Handle* handles) {
 
 extern "C" {
 SPFFT_DEFINE_C_API(spfft_, SpfftGrid, SpfftTransform, Grid, Transform, double)
+
+static_assert(SPFFT_AMD_NOALLOC_SCRATCH_BYTES == spfft::kNoAllocScratchBytes &&
+                  SPFFT_AMD_NOALLOC_MAX_TRANSFORMS == spfft::kNoAllocMaxTransforms,
+              "public limits of the allocation-free entry point");
+
+SpfftError spfft_amd_multi_transform_noalloc(int numTransforms, SpfftTransform* transforms,
+                                             const double* const* inputPointers,
+                                             double* const* outputPointers, int forward,
+                                             const SpfftScalingType* scalingTypes) {
+  return guarded([&] {
+    spfft::MultiTransformInternal<spfft::Transform>::noalloc(
+        numTransforms, transforms, inputPointers, outputPointers, forward != 0, scalingTypes);
+  });
+}
+
 SPFFT_DEFINE_C_API(spfft_float_, SpfftFloatGrid, SpfftFloatTransform, GridFloat, TransformFloat,
                    float)
 }

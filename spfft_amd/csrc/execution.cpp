@@ -707,7 +707,7 @@ template <typename T>
 bool ExecutionGPU<T>::batch_qualifies(ExecutionGPU<T>* const* ex, size_t n, const T* const* in,
                                       T* const* out, const T** din, T** dout,
                                       BatchShape* shape) {
-  if (n < 2) return false;
+  if (n < 1) return false;
   const IndexMap& m0 = *ex[0]->map_;
   for (size_t t = 0; t < n; ++t) {
     const IndexMap& m = *ex[t]->map_;
@@ -812,6 +812,7 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
                                 const std::vector<const T*>& in, const std::vector<T*>& out,
                                 bool forward, const std::vector<SpfftScalingType>& scaling) {
   const size_t n = ex.size();
+  if (n < 2) return false;
   std::vector<const T*> din(n);
   std::vector<T*> dout(n);
   BatchShape shape;
@@ -950,6 +951,38 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
   }
   check_hip(hipGraphLaunch(plan.exec, nullptr));
   return true;
+}
+
+template <typename T>
+void ExecutionGPU<T>::run_batch_noalloc(ExecutionGPU<T>* const* ex, size_t n, const T* const* in,
+                                        T* const* out, bool forward,
+                                        const SpfftScalingType* scaling, void* scratch,
+                                        size_t scratchBytes, void* stage, size_t stageBytes) {
+  if (n < 1 || (forward && !scaling)) throw InvalidParameterError();
+  // host stage: device views of the user pointers, then the image of the
+  // device tables (argument table padded to 256 bytes, then the block map)
+  const size_t viewBytes = 2 * n * sizeof(void*);
+  if (viewBytes > stageBytes) throw InvalidParameterError();
+  char* h = static_cast<char*>(stage);
+  const T** din = reinterpret_cast<const T**>(h);
+  T** dout = reinterpret_cast<T**>(h + n * sizeof(void*));
+  BatchShape shape;
+  if (!batch_qualifies(ex, n, in, out, din, dout, &shape)) throw InvalidParameterError();
+  const size_t argBytes = (n * sizeof(FastArgs<T>) + 255) / 256 * 256;
+  const size_t tableBytes = argBytes + static_cast<size_t>(shape.blocks) * sizeof(int2);
+  if (tableBytes > scratchBytes || viewBytes + tableBytes > stageBytes) {
+    throw InvalidParameterError();
+  }
+  char* image = h + viewBytes;
+  fill_batch_tables(ex, n, din, dout, forward, scaling, reinterpret_cast<FastArgs<T>*>(image),
+                    reinterpret_cast<int2*>(image + argBytes));
+  // From pageable memory the copy has consumed the stage when it returns.
+  check_hip(hipMemcpyAsync(scratch, image, tableBytes, hipMemcpyHostToDevice, nullptr));
+  const bool plane = tuned_plane_choice(shape.dimX, forward) == 1 &&
+                     plane_candidate(ex, n, din, dout, forward, shape);
+  const auto* dArgs = static_cast<const FastArgs<T>*>(scratch);
+  const auto* dMap = reinterpret_cast<const int2*>(static_cast<const char*>(scratch) + argBytes);
+  launch_batch(forward, shape, static_cast<int>(n), plane, dArgs, dMap, nullptr);
 }
 
 template class ExecutionGPU<double>;

@@ -97,6 +97,20 @@ public:
   // whole-plane, 0 class-split, -1 not decided yet.
   static int tuned_plane_choice(int n, bool forward);
 
+  // run_batch for callers that cannot allocate (n >= 1 transforms). The
+  // argument table and the z block map are written to `scratch`
+  // (scratchBytes of device memory owned by the caller and reused by every
+  // call), staged through `stage` (stageBytes of host memory); the launches
+  // go to the default stream. No device or host allocation, no graph, no
+  // timing: the plane-kernel family tuned by earlier batched calls is used,
+  // else the class-split kernels. Throws InvalidParameterError when the batch
+  // does not qualify (host pointers, mixed shapes, lengths without fixed
+  // kernels, empty sets) or its tables do not fit.
+  static void run_batch_noalloc(ExecutionGPU* const* ex, size_t n, const T* const* in,
+                                T* const* out, bool forward, const SpfftScalingType* scaling,
+                                void* scratch, size_t scratchBytes, void* stage,
+                                size_t stageBytes);
+
   T* space_domain_gpu() const { return grid_->space_gpu(); }
   T* space_domain_host() const { return grid_->space_host(); }
   hipStream_t stream() const { return stream_; }

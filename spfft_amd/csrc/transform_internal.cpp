@@ -1,6 +1,7 @@
 #include "transform_internal.hpp"
 
 #include "hip_check.hpp"
+#include "no_alloc.hpp"
 #include "spfft/exceptions.hpp"
 
 namespace spfft {
@@ -160,6 +161,33 @@ bool TransformInternal<T>::batch_forward(const std::vector<TransformInternal*>& 
   std::vector<std::shared_ptr<ExecutionGPU<T>>> execs;
   for (TransformInternal* t : ts) execs.push_back(t->exec_);
   return run_batched<T>(ts, execs, in, out, true, scaling, ts[0]->device_id());
+}
+
+template <typename T>
+void TransformInternal<T>::batch_noalloc(TransformInternal* const* ts, size_t n,
+                                         const T* const* in, T* const* out, bool forward,
+                                         const SpfftScalingType* scaling) {
+  if (n < 1 || n > static_cast<size_t>(kNoAllocMaxTransforms) || !ts || !in || !out) {
+    throw InvalidParameterError();
+  }
+  ExecutionGPU<T>* ex[kNoAllocMaxTransforms];
+  for (size_t t = 0; t < n; ++t) {
+    if (!ts[t] || !ts[t]->exec_ || ts[t]->device_id() != ts[0]->device_id()) {
+      throw InvalidParameterError();
+    }
+    ex[t] = ts[t]->exec_.get();
+  }
+  check_preceding();
+  DeviceGuard guard(ts[0]->device_id());
+  const NoAllocScratch s = noalloc_scratch(ts[0]->device_id());
+  {
+    std::lock_guard<std::mutex> lock(*s.stageLock);
+    ExecutionGPU<T>::run_batch_noalloc(ex, n, in, out, forward, scaling, s.device,
+                                       kNoAllocScratchBytes, s.stage, kNoAllocStageBytes);
+  }
+  bool sync = false;
+  for (size_t t = 0; t < n; ++t) sync |= ts[t]->execution_mode() == SPFFT_EXEC_SYNCHRONOUS;
+  if (sync) check_hip(hipStreamSynchronize(nullptr));
 }
 
 template class TransformInternal<double>;

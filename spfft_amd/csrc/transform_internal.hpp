@@ -41,6 +41,12 @@ public:
                             const std::vector<const T*>& in, const std::vector<T*>& out,
                             const std::vector<SpfftScalingType>& scaling);
 
+  // Allocation-free batched call (ExecutionGPU::run_batch_noalloc) of n
+  // GPU transforms of one device; the tables go to that device's fixed
+  // scratch. Synchronises when any transform is in synchronous mode.
+  static void batch_noalloc(TransformInternal* const* ts, size_t n, const T* const* in,
+                            T* const* out, bool forward, const SpfftScalingType* scaling);
+
   T* space_domain_data(SpfftProcessingUnitType location);
 
   SpfftTransformType type() const { return map_->type; }

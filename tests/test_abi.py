@@ -32,7 +32,7 @@ def declared_functions():
 
 def test_every_declared_symbol_is_exported():
     names = declared_functions()
-    assert len(names) == 75  # 36 double + 36 float + index-map and profiling hooks
+    assert len(names) == 76  # 36 double + 36 float + index-map, profiling and no-alloc hooks
     out = subprocess.run(["nm", "-D", "--defined-only", sp.LIB_PATH], check=True,
                          capture_output=True, text=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
@@ -214,3 +214,25 @@ def test_grid_create_argument_errors(precision):
     for dims in bad:
         assert api["grid_create"](ctypes.byref(h), *dims, -1) == \
             sp.SPFFT_INVALID_PARAMETER_ERROR, dims
+
+
+def test_noalloc_entry_point_argument_errors():
+    """The allocation-free batched entry point (include/spfft_amd/no_alloc.h)
+    rejects empty, oversized and null-handle batches and host transforms
+    before any device work."""
+    f = sp.lib.spfft_amd_multi_transform_noalloc
+    vp = ctypes.c_void_p
+    one = (vp * 1)(None)
+    sc = (ctypes.c_int * 1)(0)
+    assert f(0, one, one, one, 0, sc) == sp.SPFFT_INVALID_PARAMETER_ERROR
+    assert f(1, one, one, one, 0, sc) == sp.SPFFT_INVALID_PARAMETER_ERROR
+    many = (vp * 1025)()
+    assert f(1025, many, many, many, 1, (ctypes.c_int * 1025)()) == \
+        sp.SPFFT_INVALID_PARAMETER_ERROR
+    t = sphere_triplets(3)
+    host = sp.Transform(1, sp.SPFFT_PU_HOST, sp.SPFFT_TRANS_C2C, 8, 8, 8, len(t),
+                        sp.SPFFT_INDEX_TRIPLETS, t)
+    buf = np.zeros(2 * 8 ** 3)
+    with pytest.raises(sp.SpfftError) as e:
+        sp.multi_transform_noalloc([host], [buf], [buf], forward=False)
+    assert e.value.code == sp.SPFFT_INVALID_PARAMETER_ERROR
