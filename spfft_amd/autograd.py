@@ -12,6 +12,13 @@ grad_c = 2 B^H (V psi). With recompute=True (the default) psi is not kept
 between forward and backward: the backward recomputes it, so memory per band
 stays at one dense field regardless of the batch size.
 
+SparseEnergyBatch: the same loss summed over bands, L = sum_b sum_r V |B_b c_b|^2,
+for bands that each have their own transform (own grid), evaluated with the
+fused batched chain (multi_transform_backward for psi, multi_transform_forward
+without scaling for the adjoint). recompute="auto" keeps the bands' fields for
+the backward only when they fit a memory budget (by default a quarter of the
+device memory free at the call), else recomputes them.
+
 Tensors are complex128 (double transforms) or complex64 (float transforms) on
 the GPU; the dense field is shaped (Nz, Ny, Nx). Only C2C transforms qualify;
 shapes, dtypes, devices and the transform type are checked before any pointer
@@ -132,3 +139,85 @@ def sparse_backward(c, transform, dims):
 
 def sparse_energy(c, potential, transform, dims, recompute=True):
     return SparseEnergy.apply(c, potential, transform, dims, recompute)
+
+
+def _field_budget(memory_budget):
+    if memory_budget is not None:
+        return int(memory_budget)
+    free, _ = torch.cuda.mem_get_info()
+    return free // 4
+
+
+def _batch_backward(transforms, dims, C):
+    nx, ny, nz = dims
+    n = transforms[0].num_values
+    for t in transforms:
+        _check_transform(t, dims)
+        if t.num_values != n or _dtype(t) != _dtype(transforms[0]):
+            raise ValueError("all bands need transforms of one precision and value count")
+    if C.dim() != 2 or C.shape[0] != len(transforms):
+        raise ValueError(f"C: shape {tuple(C.shape)}, need ({len(transforms)}, {n})")
+    _check_tensor("C", C, transforms[0], len(transforms) * n)
+    C = C.contiguous()
+    psi = torch.empty((len(transforms), nz, ny, nx), dtype=C.dtype, device=C.device)
+    with _OnDefaultStream():
+        sp.multi_transform_backward(transforms, [torch.view_as_real(C[b]).data_ptr()
+                                                 for b in range(len(transforms))],
+                                    [torch.view_as_real(psi[b]).data_ptr()
+                                     for b in range(len(transforms))])
+    return psi
+
+
+def _batch_adjoint(transforms, n, fields):
+    out = torch.empty((len(transforms), n), dtype=fields.dtype, device=fields.device)
+    fields = fields.contiguous()
+    with _OnDefaultStream():
+        sp.multi_transform_forward(transforms, [torch.view_as_real(fields[b]).data_ptr()
+                                                for b in range(len(transforms))],
+                                   [torch.view_as_real(out[b]).data_ptr()
+                                    for b in range(len(transforms))],
+                                   [sp.SPFFT_NO_SCALING] * len(transforms))
+    return out
+
+
+def recompute_fields(field_bytes, recompute="auto", memory_budget=None):
+    """True when the batched energy's backward recomputes the fields: always
+    for recompute=True, never for False, and for "auto" when the fields
+    (field_bytes) exceed the budget (memory_budget bytes, or a quarter of the
+    device memory free now)."""
+    if recompute == "auto":
+        return field_bytes > _field_budget(memory_budget)
+    return bool(recompute)
+
+
+class SparseEnergyBatch(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, C, potential, transforms, dims, recompute="auto", memory_budget=None):
+        psi = _batch_backward(transforms, dims, C)
+        energy = torch.sum(potential.to(torch.float64) *
+                           torch.sum((psi.real ** 2 + psi.imag ** 2).to(torch.float64), dim=0))
+        ctx.transforms, ctx.dims = transforms, dims
+        ctx.recompute = recompute_fields(psi.numel() * psi.element_size(), recompute,
+                                         memory_budget)
+        if ctx.recompute:
+            ctx.save_for_backward(C, potential)
+        else:
+            ctx.save_for_backward(C, potential, psi)
+        return energy
+
+    @staticmethod
+    def backward(ctx, grad_energy):
+        if ctx.recompute:
+            C, potential = ctx.saved_tensors
+            psi = _batch_backward(ctx.transforms, ctx.dims, C)
+        else:
+            C, potential, psi = ctx.saved_tensors
+        g = _batch_adjoint(ctx.transforms, C.shape[1], (2.0 * potential) * psi)
+        return grad_energy.to(potential.dtype) * g, None, None, None, None, None
+
+
+def sparse_energy_batch(C, potential, transforms, dims, recompute="auto", memory_budget=None):
+    """sum_b sum_r V |psi_b|^2 over the bands' transforms (C: bands x values),
+    accumulated and returned in float64 for either precision; the gradient has
+    C's dtype."""
+    return SparseEnergyBatch.apply(C, potential, transforms, dims, recompute, memory_budget)
