@@ -261,9 +261,9 @@ std::shared_ptr<TransformTables<T>> build_tables(const std::shared_ptr<IndexMap>
     }
   }
 
-  tb.twX = DeviceBuffer<Cx<T>>(map->dimX);
-  tb.twY = DeviceBuffer<Cx<T>>(map->dimY);
-  tb.twZ = DeviceBuffer<Cx<T>>(map->dimZ);
+  tb.twX = DeviceBuffer<Tw<T>>(map->dimX);
+  tb.twY = DeviceBuffer<Tw<T>>(map->dimY);
+  tb.twZ = DeviceBuffer<Tw<T>>(map->dimZ);
   generate_twiddles<T>(tb.twX.data(), map->dimX, stream);
   generate_twiddles<T>(tb.twY.data(), map->dimY, stream);
   generate_twiddles<T>(tb.twZ.data(), map->dimZ, stream);

@@ -31,7 +31,7 @@ struct TransformTables {
   bool fast = false;      // fixed-length kernels serve this set
   int numCols = 0;
   int pitchT = 0;  // whole-plane batch kernels' tile pitch (0: not applicable)
-  DeviceBuffer<Cx<T>> twX, twY, twZ;
+  DeviceBuffer<Tw<T>> twX, twY, twZ;
   DeviceBuffer<int> entryValue, entrySlot, stickBegin, stickY, stickCol, colX, colBegin;
   DeviceBuffer<int> stickAt;
   DeviceBuffer<int2> zEntries;

@@ -96,6 +96,13 @@ constexpr int padded_len(int n) {
 }
 constexpr int make_odd(int n) { return n | 1; }
 constexpr int round_up(int n, int m) { return (n + m - 1) / m * m; }
+// Offset (in elements) of an LDS twiddle table placed after n elements:
+// aligned to a whole table entry (16 bytes), so each entry loads with one
+// instruction.
+template <typename T>
+__host__ __device__ constexpr int tw_offset(int n) {
+  return round_up(n, static_cast<int>(sizeof(Tw<T>) / sizeof(Cx<T>)));
+}
 
 // Element `i` of an array smaller than 4 GiB, addressed with a 32-bit byte
 // offset (lets the compiler use a scalar base and one offset register).
@@ -158,7 +165,7 @@ struct NoHook {
 template <typename T, int N, bool BWD, int TWS, int E = elems_per_lane(N), int NS = 1,
           int PE = 0, typename Hook = NoHook>
 __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
-                                         const Cx<T>* __restrict__ tw, Hook hook = Hook()) {
+                                         const Tw<T>* __restrict__ tw, Hook hook = Hook()) {
   constexpr int PL = N / E;
   constexpr int R = PE > 0 ? full_radix(N / NS, E) : wave_radix(N / NS, E);
   static_assert(R > 0, "no radix divides the lane's element count");
@@ -208,13 +215,15 @@ __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
 // operations instead of 18: w1 x1 + conj(w1) x3 = (c p.re - s d.im,
 // c p.im + s d.re) with p = x1 + x3, d = x1 - x3, w1 = c + i s.
 template <typename T, int R>
-__device__ __forceinline__ Cx<T> dif_sum(const Cx<T>* x, const Cx<T>* w) {
+__device__ __forceinline__ Cx<T> dif_sum(const Cx<T>* x, const Tw<T>* w) {
   if constexpr (R == 4) {
-    const T w2 = w[2].re;
+    // W_4 roots have components 0 and +-1: their table entries are exact
+    const Cx<T> w1 = tw_root(w[1]);
+    const T w2 = tw_root(w[2]).re;
     const T pr = x[1].re + x[3].re, pi = x[1].im + x[3].im;
     const T dr = x[1].re - x[3].re, di = x[1].im - x[3].im;
-    return Cx<T>{x[0].re + w2 * x[2].re + w[1].re * pr - w[1].im * di,
-                 x[0].im + w2 * x[2].im + w[1].re * pi + w[1].im * dr};
+    return Cx<T>{x[0].re + w2 * x[2].re + w1.re * pr - w1.im * di,
+                 x[0].im + w2 * x[2].im + w1.re * pi + w1.im * dr};
   } else {
     Cx<T> acc = x[0];
 #pragma unroll
@@ -247,7 +256,11 @@ struct ZShape {
   static constexpr int K = (S * NZ + NT - 1) / NT;
   // tile, one spare element that absorbs repeated triplets, twiddle table
   static constexpr int TW = S * LZ + 1;
-  static size_t lds() { return (static_cast<size_t>(TW) + NZ) * sizeof(Cx<T>); }
+  // twiddle table after the tile
+  static size_t lds() {
+    return static_cast<size_t>(tw_offset<T>(TW)) * sizeof(Cx<T>) +
+           static_cast<size_t>(NZ) * sizeof(Tw<T>);
+  }
 };
 
 // Hermitian fill of the (0,0) stick in the tile (reference
@@ -278,7 +291,7 @@ __device__ __forceinline__ void z_backward_runs(const FastArgs<T>& a, int bid) {
   using L = typename Z::L;
   constexpr int S = Z::S, LZ = Z::LZ, NT = Z::NT;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
-  Cx<T>* twl = tile + Z::TW;
+  Tw<T>* twl = reinterpret_cast<Tw<T>*>(tile + tw_offset<T>(Z::TW));
   const int s0 = bid * S;
   const int ns = min(S, a.nst - s0);
   const int lane = threadIdx.x % kWave;
@@ -332,7 +345,7 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
   using L = typename Z::L;
   constexpr int S = Z::S, LZ = Z::LZ, NT = Z::NT, K = Z::K;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
-  Cx<T>* twl = tile + Z::TW;
+  Tw<T>* twl = reinterpret_cast<Tw<T>*>(tile + tw_offset<T>(Z::TW));
   const int s0 = bid * S;
   const int ns = min(S, a.nst - s0);
   if (a.hasRuns) {
@@ -352,7 +365,7 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
     ep[k] = e < e1 && x.y >= 0 ? x.y : S * LZ;
   }
   static_assert(NZ <= NT, "one twiddle per thread");
-  const Cx<T> tw = a.twZ[threadIdx.x < NZ ? threadIdx.x : 0];
+  const Tw<T> tw = a.twZ[threadIdx.x < NZ ? threadIdx.x : 0];
   for (int t = threadIdx.x; t < S * LZ; t += NT) tile[t] = Cx<T>{T(0), T(0)};
   T vr[K], vi[K];
 #pragma unroll
@@ -403,7 +416,7 @@ __device__ __forceinline__ void z_forward_body(const FastArgs<T>& a, int bid) {
   using L = typename Z::L;
   constexpr int S = Z::S, LZ = Z::LZ, NT = Z::NT, K = Z::K;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
-  Cx<T>* twl = tile + Z::TW;
+  Tw<T>* twl = reinterpret_cast<Tw<T>*>(tile + tw_offset<T>(Z::TW));
   const int s0 = bid * S;
   const int ns = min(S, a.nst - s0);
   for (int t = threadIdx.x; t < S * NZ; t += NT) {
@@ -508,7 +521,8 @@ struct XYShape {
     return a > b ? a : b;
   }
   static size_t lds_backward(int ncol, bool rc) {
-    return (tile_backward(ncol) + N + (rc ? N : 0)) * sizeof(Cx<T>);
+    return (tw_offset<T>(static_cast<int>(tile_backward(ncol))) + (rc ? N : 0)) * sizeof(Cx<T>) +
+           static_cast<size_t>(N) * sizeof(Tw<T>);
   }
   // rows of the tile: the y lines' exchange (padded index map) reaches row
   // padded_len(N) - 1; the x pass may round the row count up to whole waves
@@ -544,12 +558,12 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
   // The twiddle table W_N (x and y have the same length) is staged in LDS
   // after the tile.
   const int tileA = S::ROWS_A * LU, tileB = S::ROWS_ALLOC * LB;
-  Cx<T>* twl = A + (tileA > tileB ? tileA : tileB);
+  Tw<T>* twl = reinterpret_cast<Tw<T>*>(A + tw_offset<T>(tileA > tileB ? tileA : tileB));
   if (threadIdx.x < N) twl[threadIdx.x] = a.twX[threadIdx.x];
   // R2C: hermitian fill of the plane's x = 0 column before its y transform
   // (reference symmetry_host.hpp:47-58, same rule as the (0,0) stick), into
   // LDS after the twiddles; the column's gathers read it from there.
-  Cx<T>* fill = twl + N;
+  Cx<T>* fill = reinterpret_cast<Cx<T>*>(twl + N);
   const bool fillCol = RC && a.colX[0] == 0;
   if (fillCol) {
     for (int y = threadIdx.x; y < N; y += S::NT) {
@@ -577,8 +591,8 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
   // lane's loads are all in flight together.
   {
     const int q = lane % LM::P;
-    Cx<T> wl[R];
-    wl[0] = Cx<T>{T(1), T(0)};
+    Tw<T> wl[R];
+    wl[0] = tw_one<T>();
 #pragma unroll
     for (int l = 1; l < R; ++l) wl[l] = twiddle<true>(twl, ((r * l) % R) * M);
     // Columns up to and including ncol, which comes out zero (its stickAt
@@ -616,7 +630,7 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
 #pragma unroll
         for (int e = 0; e < EC; ++e) {
           const int j = q + LM::P * (e0 + e);
-          const Cx<T> wj = twiddle<true>(twl, r * j);
+          const Tw<T> wj = twiddle<true>(twl, r * j);
           v[e0 + e] = cmul(dif_sum<T, R>(xv[e], wl), wj);
         }
       }
@@ -698,18 +712,18 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   // twiddles are read from the table in global memory (L1/L2 resident): the
   // tile leaves no room for a copy at two workgroups per CU
-  const Cx<T>* twl = a.twX;
+  const Tw<T>* twl = a.twX;
 
   // x: every row, decimation in frequency to the columns x = c + R m, then
   // the MX-point FFT; row y's result lands in Tb[y][m]. Class twiddles are
   // loaded once; the row loads of a lane are issued together.
   {
     const int q = lane % LM::P;
-    Cx<T> wl[R];
-    wl[0] = Cx<T>{T(1), T(0)};
+    Tw<T> wl[R];
+    wl[0] = tw_one<T>();
 #pragma unroll
     for (int l = 1; l < R; ++l) wl[l] = twiddle<false>(twl, ((c * l) % R) * MX);
-    Cx<T> wj[LM::E];
+    Tw<T> wj[LM::E];
 #pragma unroll
     for (int e = 0; e < LM::E; ++e) wj[e] = twiddle<false>(twl, c * (q + LM::P * e));
     for (int y0 = wave * LM::PER_WAVE; y0 < N; y0 += NW * LM::PER_WAVE) {
@@ -805,8 +819,8 @@ struct PlaneShape {
   // column) land on distinct bank groups
   static int pitch(int ncol) { return round_up(ncol + 1, YL::PER_WAVE) | 1; }
   static size_t lds(int ncol) {
-    return (static_cast<size_t>(ROWS) * pitch(ncol) + static_cast<size_t>(XROWS) * LB + N) *
-           sizeof(Cx<T>);
+    return static_cast<size_t>(tw_offset<T>(ROWS * pitch(ncol) + XROWS * LB)) * sizeof(Cx<T>) +
+           static_cast<size_t>(N) * sizeof(Tw<T>);
   }
 };
 
@@ -838,7 +852,7 @@ __device__ __forceinline__ void plane_backward_body(const FastArgs<T>& a, int z)
   const int PU = a.pitchT, ncol = a.ncol;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
   Cx<T>* xs = tile + S::ROWS * PU;
-  Cx<T>* twl = xs + S::XROWS * LB;
+  Tw<T>* twl = reinterpret_cast<Tw<T>*>(tile + tw_offset<T>(S::ROWS * PU + S::XROWS * LB));
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   if (threadIdx.x < N) twl[threadIdx.x] = a.twX[threadIdx.x];
   // column of each x this lane handles in the x pass (ncol = zero column)
@@ -901,7 +915,7 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
   const int PU = a.pitchT, ncol = a.ncol;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
   Cx<T>* xs = tile + S::ROWS * PU;
-  Cx<T>* twl = xs + S::XROWS * LB;
+  Tw<T>* twl = reinterpret_cast<Tw<T>*>(tile + tw_offset<T>(S::ROWS * PU + S::XROWS * LB));
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   if (threadIdx.x < N) twl[threadIdx.x] = a.twX[threadIdx.x];
   int uu[XL::E];
@@ -1000,7 +1014,8 @@ struct FullShape {
   static constexpr bool PERSISTENT = sizeof(T) == 4;
   // tile, twiddle table, x -> compact column map of the current plane
   static constexpr size_t lds() {
-    return (static_cast<size_t>(ROWS) * PU + N) * sizeof(Cx<T>) + N * sizeof(int);
+    return static_cast<size_t>(tw_offset<T>(ROWS * PU)) * sizeof(Cx<T>) + N * sizeof(Tw<T>) +
+           N * sizeof(int);
   }
 };
 
@@ -1183,7 +1198,7 @@ __global__ void __launch_bounds__((FullShape<T, N>::NT))
   using L = typename S::L;
   constexpr int E = L::E, P = L::P, PU = S::PU, ROWS = S::ROWS;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
-  Cx<T>* twl = tile + ROWS * PU;
+  Tw<T>* twl = reinterpret_cast<Tw<T>*>(tile + tw_offset<T>(ROWS * PU));
   int* xmap = reinterpret_cast<int*>(twl + N);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int q = lane % P;
@@ -1292,7 +1307,7 @@ __global__ void __launch_bounds__((FullShape<T, N>::NT))
   using L = typename S::L;
   constexpr int E = L::E, P = L::P, PU = S::PU;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
-  Cx<T>* twl = tile + S::ROWS * PU;
+  Tw<T>* twl = reinterpret_cast<Tw<T>*>(tile + tw_offset<T>(S::ROWS * PU));
   int* xmap = reinterpret_cast<int*>(twl + N);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int q = lane % P;

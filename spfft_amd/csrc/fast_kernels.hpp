@@ -60,9 +60,9 @@ struct FastArgs {
   const int* resBegin = nullptr;      // R + 1 offsets into resSticks
   const int* resYM = nullptr;         // per resSticks entry: y << 16 | (x / R)
   int pitchT = 0;                     // whole-plane kernels: row pitch of the plane tile
-  const Cx<T>* twX = nullptr;
-  const Cx<T>* twY = nullptr;
-  const Cx<T>* twZ = nullptr;
+  const Tw<T>* twX = nullptr;
+  const Tw<T>* twY = nullptr;
+  const Tw<T>* twZ = nullptr;
   T scale = T(1);
   int useScale = 0;
 };

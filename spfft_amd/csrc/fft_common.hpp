@@ -38,10 +38,59 @@ template <bool BWD, typename T>
 __device__ __forceinline__ Cx<T> rot(Cx<T> a) {
   return BWD ? Cx<T>{-a.im, a.re} : Cx<T>{a.im, -a.re};
 }
-template <bool BWD, typename T>
-__device__ __forceinline__ Cx<T> twiddle(const Cx<T>* __restrict__ w, int t) {
-  const Cx<T> v = w[t];
+// ---- twiddles (table entries Tw<T>: the root, or a compensated FP32 pair)
+
+template <typename T>
+__device__ __forceinline__ TwPair<T> cconj(TwPair<T> a) {
+  return {cconj(a.hi), cconj(a.lo)};
+}
+template <bool BWD, typename W>
+__device__ __forceinline__ W twiddle(const W* __restrict__ w, int t) {
+  const W v = w[t];
   return BWD ? cconj(v) : v;
+}
+// a (hi + lo): the small lo product is formed first and enters the hi
+// product's fused multiply-adds.
+template <typename T>
+__device__ __forceinline__ Cx<T> cmul(Cx<T> a, TwPair<T> b) {
+  const T lr = a.re * b.lo.re - a.im * b.lo.im;
+  const T li = a.re * b.lo.im + a.im * b.lo.re;
+  return {fma(a.re, b.hi.re, fma(-a.im, b.hi.im, lr)), fma(a.re, b.hi.im, fma(a.im, b.hi.re, li))};
+}
+// The root part of a table entry (exact for roots whose components are 0 or
+// +-1, where lo is zero).
+template <typename T>
+__device__ __forceinline__ Cx<T> tw_root(Cx<T> w) {
+  return w;
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> tw_root(TwPair<T> w) {
+  return w.hi;
+}
+template <typename T>
+__device__ __forceinline__ Tw<T> tw_one() {
+  if constexpr (sizeof(T) == 4) {
+    return Tw<T>{Cx<T>{T(1), T(0)}, Cx<T>{T(0), T(0)}};
+  } else {
+    return Tw<T>{T(1), T(0)};
+  }
+}
+
+// x k and x (c + i s) for constants written in double, rounded once to T
+// (the butterflies' fixed roots; the twiddle tables carry the compensated
+// roots).
+template <typename T>
+__device__ __forceinline__ T kmul(T x, double k) {
+  return x * static_cast<T>(k);
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> ckmul(Cx<T> x, double c, double s) {
+  const T cc = static_cast<T>(c), ss = static_cast<T>(s);
+  return {x.re * cc - x.im * ss, x.re * ss + x.im * cc};
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> kscale(Cx<T> a, double k) {
+  return {kmul(a.re, k), kmul(a.im, k)};
 }
 
 // ---- butterflies: in-register DFT of length R with sign -1 (fwd) / +1 (bwd)
@@ -49,7 +98,7 @@ __device__ __forceinline__ Cx<T> twiddle(const Cx<T>* __restrict__ w, int t) {
 template <typename T, int R, bool BWD>
 struct Butterfly {
   // Direct DFT from the length-n table: W_R^m = W_n^(m n / R).
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>* __restrict__ w, int n) {
+  __device__ __forceinline__ static void run(Cx<T>* v, const Tw<T>* __restrict__ w, int n) {
     Cx<T> out[R];
     const int step = n / R;
 #pragma unroll
@@ -66,7 +115,7 @@ struct Butterfly {
 
 template <typename T, bool BWD>
 struct Butterfly<T, 2, BWD> {
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
+  __device__ __forceinline__ static void run(Cx<T>* v, const Tw<T>*, int) {
     const Cx<T> a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
@@ -75,7 +124,7 @@ struct Butterfly<T, 2, BWD> {
 
 template <typename T, bool BWD>
 struct Butterfly<T, 4, BWD> {
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
+  __device__ __forceinline__ static void run(Cx<T>* v, const Tw<T>*, int) {
     const Cx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
     const Cx<T> t2 = cadd(v[1], v[3]), t3 = rot<BWD>(csub(v[1], v[3]));
     v[0] = cadd(t0, t2);
@@ -87,18 +136,18 @@ struct Butterfly<T, 4, BWD> {
 
 template <typename T, bool BWD>
 struct Butterfly<T, 8, BWD> {
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>* w, int n) {
+  __device__ __forceinline__ static void run(Cx<T>* v, const Tw<T>* w, int n) {
     Cx<T> e[4] = {v[0], v[2], v[4], v[6]};
     Cx<T> o[4] = {v[1], v[3], v[5], v[7]};
     Butterfly<T, 4, BWD>::run(e, w, n);
     Butterfly<T, 4, BWD>::run(o, w, n);
-    const T h = T(0.70710678118654752440);
+    constexpr double h = 0.70710678118654752440;
     // o[k] *= exp(-/+ 2 pi i k / 8)
-    const Cx<T> o1 = BWD ? Cx<T>{h * (o[1].re - o[1].im), h * (o[1].re + o[1].im)}
-                         : Cx<T>{h * (o[1].re + o[1].im), h * (o[1].im - o[1].re)};
+    const Cx<T> o1 = BWD ? Cx<T>{kmul(o[1].re - o[1].im, h), kmul(o[1].re + o[1].im, h)}
+                         : Cx<T>{kmul(o[1].re + o[1].im, h), kmul(o[1].im - o[1].re, h)};
     const Cx<T> o2 = rot<BWD>(o[2]);
-    const Cx<T> o3 = BWD ? Cx<T>{-h * (o[3].re + o[3].im), h * (o[3].re - o[3].im)}
-                         : Cx<T>{h * (o[3].im - o[3].re), -h * (o[3].re + o[3].im)};
+    const Cx<T> o3 = BWD ? Cx<T>{kmul(o[3].re + o[3].im, -h), kmul(o[3].re - o[3].im, h)}
+                         : Cx<T>{kmul(o[3].im - o[3].re, h), kmul(o[3].re + o[3].im, -h)};
     v[0] = cadd(e[0], o[0]);
     v[4] = csub(e[0], o[0]);
     v[1] = cadd(e[1], o1);
@@ -114,7 +163,7 @@ struct Butterfly<T, 8, BWD> {
 // the length-8 DFTs of the even and odd inputs.
 template <typename T, bool BWD>
 struct Butterfly<T, 16, BWD> {
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>* w, int n) {
+  __device__ __forceinline__ static void run(Cx<T>* v, const Tw<T>* w, int n) {
     Cx<T> e[8], o[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -124,19 +173,19 @@ struct Butterfly<T, 16, BWD> {
     Butterfly<T, 8, BWD>::run(e, w, n);
     Butterfly<T, 8, BWD>::run(o, w, n);
     // cos and sin of 2 pi k / 16, k = 0..7
-    constexpr T c[8] = {T(1), T(0.92387953251128675613), T(0.70710678118654752440),
-                        T(0.38268343236508977173), T(0), T(-0.38268343236508977173),
-                        T(-0.70710678118654752440), T(-0.92387953251128675613)};
-    constexpr T s[8] = {T(0), T(0.38268343236508977173), T(0.70710678118654752440),
-                        T(0.92387953251128675613), T(1), T(0.92387953251128675613),
-                        T(0.70710678118654752440), T(0.38268343236508977173)};
+    constexpr double c[8] = {1.0, 0.92387953251128675613, 0.70710678118654752440,
+                             0.38268343236508977173, 0.0, -0.38268343236508977173,
+                             -0.70710678118654752440, -0.92387953251128675613};
+    constexpr double s[8] = {0.0, 0.38268343236508977173, 0.70710678118654752440,
+                             0.92387953251128675613, 1.0, 0.92387953251128675613,
+                             0.70710678118654752440, 0.38268343236508977173};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       Cx<T> t = o[k];
       if (k == 4) {
         t = rot<BWD>(t);
       } else if (k != 0) {
-        t = cmul(t, Cx<T>{c[k], BWD ? s[k] : -s[k]});
+        t = ckmul(t, c[k], BWD ? s[k] : -s[k]);
       }
       v[k] = cadd(e[k], t);
       v[k + 8] = csub(e[k], t);
@@ -148,17 +197,17 @@ struct Butterfly<T, 16, BWD> {
 // X[k1 + 8 k2] = sum_n2 W3^(n2 k2) W24^(n2 k1) A[n2][k1].
 template <typename T, bool BWD>
 struct Butterfly<T, 24, BWD> {
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>* w, int n);
+  __device__ __forceinline__ static void run(Cx<T>* v, const Tw<T>* w, int n);
 };
 
 template <typename T, bool BWD>
 struct Butterfly<T, 3, BWD> {
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
-    const T s3 = T(0.86602540378443864676);  // sin(2 pi / 3)
+  __device__ __forceinline__ static void run(Cx<T>* v, const Tw<T>*, int) {
+    constexpr double s3 = 0.86602540378443864676;  // sin(2 pi / 3)
     const Cx<T> t1 = cadd(v[1], v[2]);
     const Cx<T> t2 = csub(v[1], v[2]);
     const Cx<T> m = csub(v[0], cscale(t1, T(0.5)));
-    const Cx<T> r = rot<BWD>(cscale(t2, s3));
+    const Cx<T> r = rot<BWD>(kscale(t2, s3));
     v[0] = cadd(v[0], t1);
     v[1] = cadd(m, r);
     v[2] = csub(m, r);
@@ -167,17 +216,17 @@ struct Butterfly<T, 3, BWD> {
 
 template <typename T, bool BWD>
 struct Butterfly<T, 5, BWD> {
-  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>*, int) {
-    const T c1 = T(0.30901699437494742410);   // cos(2 pi / 5)
-    const T c2 = T(-0.80901699437494742410);  // cos(4 pi / 5)
-    const T s1 = T(0.95105651629515357212);   // sin(2 pi / 5)
-    const T s2 = T(0.58778525229247312917);   // sin(4 pi / 5)
+  __device__ __forceinline__ static void run(Cx<T>* v, const Tw<T>*, int) {
+    constexpr double c1 = 0.30901699437494742410;   // cos(2 pi / 5)
+    constexpr double c2 = -0.80901699437494742410;  // cos(4 pi / 5)
+    constexpr double s1 = 0.95105651629515357212;   // sin(2 pi / 5)
+    constexpr double s2 = 0.58778525229247312917;   // sin(4 pi / 5)
     const Cx<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
     const Cx<T> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-    const Cx<T> b1 = cadd(v[0], cadd(cscale(t1, c1), cscale(t2, c2)));
-    const Cx<T> b2 = cadd(v[0], cadd(cscale(t1, c2), cscale(t2, c1)));
-    const Cx<T> d1 = rot<BWD>(cadd(cscale(t3, s1), cscale(t4, s2)));
-    const Cx<T> d2 = rot<BWD>(csub(cscale(t3, s2), cscale(t4, s1)));
+    const Cx<T> b1 = cadd(v[0], cadd(kscale(t1, c1), kscale(t2, c2)));
+    const Cx<T> b2 = cadd(v[0], cadd(kscale(t1, c2), kscale(t2, c1)));
+    const Cx<T> d1 = rot<BWD>(cadd(kscale(t3, s1), kscale(t4, s2)));
+    const Cx<T> d2 = rot<BWD>(csub(kscale(t3, s2), kscale(t4, s1)));
     v[0] = cadd(v[0], cadd(t1, t2));
     v[1] = cadd(b1, d1);
     v[4] = csub(b1, d1);
@@ -187,7 +236,7 @@ struct Butterfly<T, 5, BWD> {
 };
 
 template <typename T, bool BWD>
-__device__ __forceinline__ void Butterfly<T, 24, BWD>::run(Cx<T>* v, const Cx<T>* w, int n) {
+__device__ __forceinline__ void Butterfly<T, 24, BWD>::run(Cx<T>* v, const Tw<T>* w, int n) {
   Cx<T> a[3][8];
 #pragma unroll
   for (int n2 = 0; n2 < 3; ++n2) {
@@ -196,18 +245,18 @@ __device__ __forceinline__ void Butterfly<T, 24, BWD>::run(Cx<T>* v, const Cx<T>
     Butterfly<T, 8, BWD>::run(a[n2], w, n);
   }
   // cos and sin of 2 pi m / 24, m = 0..14
-  constexpr T c15 = T(0.96592582628906828675), s15 = T(0.25881904510252076235);
-  constexpr T c30 = T(0.86602540378443864676), c45 = T(0.70710678118654752440);
-  constexpr T c[15] = {T(1),  c15,  c30,  c45,  T(0.5),  s15,  T(0),    -s15,
-                       T(-0.5), -c45, -c30, -c15, T(-1),   -c15, -c30};
-  constexpr T s[15] = {T(0), s15, T(0.5), c45, c30, c15, T(1), c15,
-                       c30,  c45, T(0.5), s15, T(0), -s15, T(-0.5)};
+  constexpr double c15 = 0.96592582628906828675, s15 = 0.25881904510252076235;
+  constexpr double c30 = 0.86602540378443864676, c45 = 0.70710678118654752440;
+  constexpr double c[15] = {1.0,  c15,  c30,  c45,  0.5,  s15,  0.0,  -s15,
+                            -0.5, -c45, -c30, -c15, -1.0, -c15, -c30};
+  constexpr double s[15] = {0.0, s15, 0.5, c45, c30, c15, 1.0, c15,
+                            c30, c45, 0.5, s15, 0.0, -s15, -0.5};
 #pragma unroll
   for (int n2 = 1; n2 < 3; ++n2) {
 #pragma unroll
     for (int k1 = 1; k1 < 8; ++k1) {
       const int m = n2 * k1;
-      a[n2][k1] = cmul(a[n2][k1], Cx<T>{c[m], BWD ? s[m] : -s[m]});
+      a[n2][k1] = ckmul(a[n2][k1], c[m], BWD ? s[m] : -s[m]);
     }
   }
 #pragma unroll

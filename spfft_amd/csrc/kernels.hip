@@ -37,7 +37,7 @@ constexpr size_t kLdsMax = 160 * 1024;
 template <typename T, int R, bool BWD>
 __device__ __forceinline__ void stockham_pass(const Cx<T>* __restrict__ src,
                                               Cx<T>* __restrict__ dst, int n, int Ns, int batch,
-                                              const Cx<T>* __restrict__ w) {
+                                              const Tw<T>* __restrict__ w) {
   const int m = n / R;
   const int twStep = n / (Ns * R);
   const int total = batch * m;
@@ -65,7 +65,7 @@ __device__ __forceinline__ void stockham_pass(const Cx<T>* __restrict__ src,
 template <typename T, bool BWD>
 __device__ __forceinline__ void stockham_pass_any(const Cx<T>* __restrict__ src,
                                                   Cx<T>* __restrict__ dst, int n, int R, int Ns,
-                                                  int batch, const Cx<T>* __restrict__ w) {
+                                                  int batch, const Tw<T>* __restrict__ w) {
   const int m = n / R;
   const int twStep = n / (Ns * R);
   const int total = batch * n;
@@ -90,7 +90,7 @@ __device__ __forceinline__ void stockham_pass_any(const Cx<T>* __restrict__ src,
 // holds the result. Caller synchronises before the call.
 template <typename T, bool BWD>
 __device__ Cx<T>* lds_fft(Cx<T>* a, Cx<T>* b, const FftDesc& f, int batch,
-                          const Cx<T>* __restrict__ w) {
+                          const Tw<T>* __restrict__ w) {
   int Ns = 1;
   for (int p = 0; p < f.npass; ++p) {
     const int R = f.radix[p];
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(kBlock) k_z_forward(StageArgs<T> a) {
 }
 
 template <typename T>
-__global__ void k_twiddles(Cx<T>* tw, int n) {
+__global__ void k_twiddles(Tw<T>* tw, int n) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   // Angle reduced to (-pi, pi] so that the argument of sincospi is exact up
@@ -311,7 +311,14 @@ __global__ void k_twiddles(Cx<T>* tw, int n) {
   const int tt = (2 * t > n) ? t - n : t;
   double s, c;
   sincospi(-2.0 * static_cast<double>(tt) / static_cast<double>(n), &s, &c);
-  tw[t] = Cx<T>{static_cast<T>(c), static_cast<T>(s)};
+  if constexpr (sizeof(T) == 4) {
+    // compensated pair: the FP64 root is hi + lo to about 2^-48
+    const float ch = static_cast<float>(c), sh = static_cast<float>(s);
+    tw[t] = Tw<T>{Cx<float>{ch, sh}, Cx<float>{static_cast<float>(c - static_cast<double>(ch)),
+                                               static_cast<float>(s - static_cast<double>(sh))}};
+  } else {
+    tw[t] = Tw<T>{static_cast<T>(c), static_cast<T>(s)};
+  }
 }
 
 // Raise the dynamic LDS limit of a kernel once per process.
@@ -368,7 +375,7 @@ int stage_batch(int n, int count, size_t* ldsBytes) {
 }
 
 template <typename T>
-void generate_twiddles(Cx<T>* tw, int n, hipStream_t stream) {
+void generate_twiddles(Tw<T>* tw, int n, hipStream_t stream) {
   const int blocks = (n + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_twiddles<T>, dim3(blocks), dim3(kBlock), 0, stream, tw, n);
   check_launch();
@@ -403,7 +410,7 @@ SPFFT_STAGE_LAUNCHER(launch_z_forward, k_z_forward, nz, a.nst,
 
 #define SPFFT_INSTANTIATE(T)                                                      \
   template int stage_batch<T>(int, int, size_t*);                                \
-  template void generate_twiddles<T>(Cx<T>*, int, hipStream_t);                  \
+  template void generate_twiddles<T>(Tw<T>*, int, hipStream_t);                  \
   template void launch_z_backward<T>(const StageArgs<T>&, hipStream_t);          \
   template void launch_y_backward<T>(const StageArgs<T>&, hipStream_t);          \
   template void launch_x_backward<T>(const StageArgs<T>&, hipStream_t);          \

@@ -20,6 +20,26 @@ struct alignas(2 * sizeof(T)) Cx {
   T re, im;
 };
 
+// Twiddle table entry exp(-2 pi i t / n), generated in FP64 on the GPU. FP64
+// transforms keep the root rounded once; FP32 transforms keep it as a
+// compensated pair hi + lo with hi = fl32(w) and lo = fl32(w - hi), so a
+// product with it carries the root to about 2^-48 and only the product's
+// own rounding remains.
+template <typename T>
+struct alignas(2 * sizeof(Cx<T>)) TwPair {
+  Cx<T> hi, lo;
+};
+template <typename T>
+struct TwSelect {
+  using type = Cx<T>;
+};
+template <>
+struct TwSelect<float> {
+  using type = TwPair<float>;
+};
+template <typename T>
+using Tw = typename TwSelect<T>::type;
+
 constexpr int kMaxPasses = 24;
 
 // Mixed-radix Stockham schedule for one FFT length.
@@ -40,7 +60,7 @@ struct StageArgs {
   int r2c = 0;
   int zeroZeroStick = 0;
   FftDesc fft;                 // schedule of the stage's FFT length
-  const Cx<T>* tw = nullptr;   // exp(-2 pi i t / n), t in [0, n)
+  const Tw<T>* tw = nullptr;   // exp(-2 pi i t / n), t in [0, n)
   const int* entryValue = nullptr;
   const int* entrySlot = nullptr;
   const int* stickBegin = nullptr;
@@ -61,7 +81,7 @@ template <typename T>
 int stage_batch(int n, int count, size_t* ldsBytes);
 
 template <typename T>
-void generate_twiddles(Cx<T>* tw, int n, hipStream_t stream);
+void generate_twiddles(Tw<T>* tw, int n, hipStream_t stream);
 
 template <typename T>
 void launch_z_backward(const StageArgs<T>& a, hipStream_t stream);
