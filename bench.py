@@ -126,7 +126,7 @@ def _bench_transforms(sp, torch, np, trs, ins, outs, steps, r2c=False):
         sp.multi_transform_backward(trs, ins, locs)
         sp.multi_transform_forward(trs, locs, outs, scal)
 
-    for _ in range(3):
+    for _ in range(5):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -180,7 +180,9 @@ def batched_bench(args, sp, torch, np):
                     ins.append(torch.from_numpy(v.view(np.float64).astype(real)).cuda())
                     outs.append(torch.empty(2 * len(t), dtype=ins[-1].dtype, device="cuda"))
                     trs.append(tr)
-            steps = max(5, args.steps // len(trs))
+            # at least 20 calls (tens of ms of GPU work), so that clock ramps and
+            # first-call costs (tuning, graph capture) stay out of the average
+            steps = max(20, args.steps // len(trs))
             sec = _bench_transforms(sp, torch, np, trs, ins, outs, steps)
             rb = np.dtype(real).itemsize
             nv = sum(len(t) for t in sets) * bands

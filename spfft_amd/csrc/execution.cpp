@@ -76,7 +76,8 @@ bool disable_fast_path() {
 
 // Choice of plane kernels for batches: SPFFT_AMD_PLANE=0 keeps the
 // class-split kernels, =1 forces the whole-plane kernels where they apply;
-// unset, the autotuner decides. Returns -1 (tune), 0 or 1.
+// unset, the autotuner decides. Returns -1 (tune), 0 or 1. A forced family
+// skips the tuner (defaults for the other choices, nothing recorded).
 int plane_kernels_forced() {
   const char* v = std::getenv("SPFFT_AMD_PLANE");
   if (v && v[0] == '0') return 0;
@@ -84,21 +85,33 @@ int plane_kernels_forced() {
   return -1;
 }
 
-// Autotuner decisions between the class-split and the whole-plane kernels,
-// per (value bytes, length, direction): timed once, on the first qualifying
-// batch, on that batch's own buffers. SPFFT_AMD_TUNE_LOG=1 prints them.
-struct TuneTable {
-  std::mutex mutex;
-  std::map<std::tuple<int, int, int>, int> choice;
+// Mean time of `reps` launches after one warm-up launch, in milliseconds.
+template <typename F>
+float time_launches(F&& launch, hipEvent_t e0, hipEvent_t e1, int reps = 3) {
+  launch();
+  check_hip(hipEventRecord(e0, nullptr));
+  for (int k = 0; k < reps; ++k) launch();
+  check_hip(hipEventRecord(e1, nullptr));
+  check_hip(hipEventSynchronize(e1));
+  float ms = 0.f;
+  check_hip(hipEventElapsedTime(&ms, e0, e1));
+  return ms / static_cast<float>(reps);
+}
+
+// Two timing events for one tuning session.
+struct TuneEvents {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  TuneEvents() {
+    check_hip(hipEventCreate(&e0));
+    check_hip(hipEventCreate(&e1));
+  }
+  ~TuneEvents() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  TuneEvents(const TuneEvents&) = delete;
+  TuneEvents& operator=(const TuneEvents&) = delete;
 };
-TuneTable& tune_table() {
-  static TuneTable* t = new TuneTable();
-  return *t;
-}
-bool tune_log() {
-  const char* v = std::getenv("SPFFT_AMD_TUNE_LOG");
-  return v && v[0] == '1';
-}
 
 // SPFFT_AMD_GRAPHS=0 replays batched calls with direct launches instead of
 // captured HIP graphs.
@@ -119,7 +132,7 @@ struct BatchPlan {
   void* map = nullptr;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
-  bool plane = false;  // plane kernels the graph was captured with
+  TuneChoice choice;  // kernel choices the graph was captured with
 };
 
 // Most recently used first. Heap-allocated and never destroyed: plans hold
@@ -432,6 +445,42 @@ FastArgs<T> ExecutionGPU<T>::fast_args() const {
 }
 
 template <typename T>
+bool ExecutionGPU<T>::z_runs_choice(bool forward, const FastArgs<T>& f, hipStream_t stream) {
+  if (!f.hasRuns) return false;
+  // batched fallbacks on the transform's own stream keep the default
+  if (stream != nullptr) return true;
+  const TuneKey key = make_tune_key(grid_->device_id(), static_cast<int>(sizeof(T)), map_->dimX,
+                                    map_->dimZ, map_->type == SPFFT_TRANS_R2C, forward, numCols_, 1);
+  TuneChoice c;
+  if (tune_lookup(key, &c)) return c.zRuns != 0;
+  // The z launch is idempotent: backward reads the values and writes the
+  // [z][stick] buffer, forward reads that buffer (already written by this
+  // call's plane launches) and writes the values.
+  TuneEvents ev;
+  FastArgs<T> a = f;
+  auto z = [&] {
+    if (forward) {
+      launch_fast_z_forward<T>(map_->dimZ, a, nullptr);
+    } else {
+      launch_fast_z_backward<T>(map_->dimZ, a, nullptr);
+    }
+  };
+  a.hasRuns = 1;
+  const float runs = time_launches(z, ev.e0, ev.e1);
+  a.hasRuns = 0;
+  const float entries = time_launches(z, ev.e0, ev.e1);
+  c.zRuns = runs <= entries ? 1 : 0;
+  tune_record(key, c);
+  if (tune_log_enabled()) {
+    std::fprintf(stderr, "spfft_amd tune: %s N=%d Nz=%d %s %s, single transform: z runs %.4f / "
+                 "entries %.4f ms -> zruns=%d\n", sizeof(T) == 8 ? "double" : "float",
+                 map_->dimX, map_->dimZ, map_->type == SPFFT_TRANS_R2C ? "R2C" : "C2C",
+                 forward ? "forward" : "backward", runs, entries, c.zRuns);
+  }
+  return c.zRuns != 0;
+}
+
+template <typename T>
 bool ExecutionGPU<T>::device_accessible(const void* ptr) {
   const void* view = nullptr;
   return classify(ptr, &view) == MemKind::Device;
@@ -564,6 +613,7 @@ void ExecutionGPU<T>::backward(const T* input, T* output, bool concurrent) {
     FastArgs<T> f = fast_args();
     f.in = inGPU;
     f.out = outGPU;
+    f.hasRuns = z_runs_choice(false, f, stream) ? 1 : 0;
     launch_fast_z_backward<T>(map_->dimZ, f, stream);
     // host output: the plane stage runs in chunks of planes, each copied out
     // while the next is computed
@@ -653,6 +703,7 @@ void ExecutionGPU<T>::forward(const T* input, T* output, SpfftScalingType scalin
     }
     f.zBegin = 0;
     f.nz = map_->dimZ;
+    f.hasRuns = z_runs_choice(true, f, stream) ? 1 : 0;
     launch_fast_z_forward<T>(map_->dimZ, f, stream);
   } else {
     if (hostIn) {
@@ -728,13 +779,16 @@ bool ExecutionGPU<T>::batch_qualifies(ExecutionGPU<T>* const* ex, size_t n, cons
   fast_z_geometry<T>(m0.dimZ, &S, &LZ, &shift);
   shape->blocks = 0;
   shape->maxNcol = 0;
+  shape->runs = true;
   for (size_t t = 0; t < n; ++t) {
     shape->blocks += (ex[t]->map_->num_sticks() + S - 1) / S;
     shape->maxNcol = std::max(shape->maxNcol, ex[t]->numCols_);
+    shape->runs = shape->runs && ex[t]->tab_->zInfo.data() != nullptr;
   }
   shape->r2c = m0.type == SPFFT_TRANS_R2C;
   shape->dimX = m0.dimX;
   shape->dimZ = m0.dimZ;
+  shape->device = ex[0]->grid_->device_id();
   return true;
 }
 
@@ -777,34 +831,85 @@ void ExecutionGPU<T>::fill_batch_tables(ExecutionGPU<T>* const* ex, size_t n, co
 }
 
 template <typename T>
-void ExecutionGPU<T>::launch_batch(bool forward, const BatchShape& shape, int n, bool plane,
-                                   const FastArgs<T>* dArgs, const int2* dMap,
-                                   hipStream_t stream) {
+void ExecutionGPU<T>::launch_batch(bool forward, const BatchShape& shape, int n,
+                                   const TuneChoice& choice, const FastArgs<T>* dArgs,
+                                   const int2* dMap, hipStream_t stream) {
   auto launch_xy = [&] {
-    if (plane) {
-      launch_plane_xy_batch<T>(forward, shape.dimX, dArgs, n, shape.dimZ, shape.maxNcol, stream);
+    if (choice.plane) {
+      launch_plane_xy_batch<T>(forward, shape.dimX, dArgs, n, shape.dimZ, shape.maxNcol,
+                               choice.slots, stream);
     } else {
       launch_fast_xy_batch<T>(forward, shape.r2c, shape.dimX, dArgs, n, shape.dimZ,
                               shape.maxNcol, stream);
     }
   };
+  const bool runs = choice.zRuns != 0;
   if (forward) {
     launch_xy();
-    launch_fast_z_batch<T>(true, shape.r2c, shape.dimZ, dArgs, dMap, shape.blocks, stream);
+    launch_fast_z_batch<T>(true, shape.r2c, shape.dimZ, dArgs, dMap, shape.blocks, runs, stream);
   } else {
-    launch_fast_z_batch<T>(false, shape.r2c, shape.dimZ, dArgs, dMap, shape.blocks, stream);
+    launch_fast_z_batch<T>(false, shape.r2c, shape.dimZ, dArgs, dMap, shape.blocks, runs, stream);
     launch_xy();
   }
 }
 
 template <typename T>
-int ExecutionGPU<T>::tuned_plane_choice(int n, bool forward) {
-  const int forced = plane_kernels_forced();
-  if (forced >= 0) return forced;
-  TuneTable& tt = tune_table();
-  std::lock_guard<std::mutex> lock(tt.mutex);
-  auto it = tt.choice.find(std::make_tuple(static_cast<int>(sizeof(T)), n, forward ? 1 : 0));
-  return it == tt.choice.end() ? -1 : it->second;
+TuneChoice ExecutionGPU<T>::tune_batch(bool forward, const BatchShape& shape, int n, bool planeOk,
+                                       const FastArgs<T>* dArgs, const int2* dMap) {
+  TuneChoice c;
+  TuneEvents ev;
+  auto z = [&](bool runs) {
+    launch_fast_z_batch<T>(forward, shape.r2c, shape.dimZ, dArgs, dMap, shape.blocks, runs,
+                           nullptr);
+  };
+  auto xy_split = [&] {
+    launch_fast_xy_batch<T>(forward, shape.r2c, shape.dimX, dArgs, n, shape.dimZ, shape.maxNcol,
+                            nullptr);
+  };
+  auto xy_plane = [&](int slots) {
+    launch_plane_xy_batch<T>(forward, shape.dimX, dArgs, n, shape.dimZ, shape.maxNcol, slots,
+                             nullptr);
+  };
+  // The forward z kernel reads the [z][stick] buffer that the xy kernel
+  // writes, the backward xy kernel the one that the z kernel writes.
+  if (forward) xy_split();
+  float zr = 0.f, ze = 0.f;
+  if (shape.runs) {
+    zr = time_launches([&] { z(true); }, ev.e0, ev.e1);
+    ze = time_launches([&] { z(false); }, ev.e0, ev.e1);
+    c.zRuns = zr <= ze ? 1 : 0;
+  } else if (!forward) {
+    z(false);
+  }
+  float split = 0.f, plane0 = 0.f, plane1 = 0.f;
+  if (planeOk) {
+    split = time_launches(xy_split, ev.e0, ev.e1);
+    plane0 = time_launches([&] { xy_plane(0); }, ev.e0, ev.e1);
+    float best = split;
+    if (plane0 < best) {
+      best = plane0;
+      c.plane = 1;
+    }
+    // persistent kernels with more than one resident workgroup per CU: also
+    // one per CU
+    if (plane_xy_slots<T>(forward, shape.dimX) > 1) {
+      plane1 = time_launches([&] { xy_plane(1); }, ev.e0, ev.e1);
+      if (plane1 < best) {
+        c.plane = 1;
+        c.slots = 1;
+      }
+    }
+  }
+  if (tune_log_enabled()) {
+    std::fprintf(stderr,
+                 "spfft_amd tune: %s N=%d Nz=%d %s %s, %d transforms, %d columns: z runs %.3f / "
+                 "entries %.3f ms; class-split %.3f, whole-plane %.3f, one per CU %.3f ms -> "
+                 "plane=%d slots=%d zruns=%d\n",
+                 sizeof(T) == 8 ? "double" : "float", shape.dimX, shape.dimZ,
+                 shape.r2c ? "R2C" : "C2C", forward ? "forward" : "backward", n, shape.maxNcol,
+                 zr, ze, split, plane0, plane1, c.plane, c.slots, c.zRuns);
+  }
+  return c;
 }
 
 template <typename T>
@@ -834,7 +939,6 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
   }
 
   const bool planeOk = plane_candidate(ex.data(), n, din.data(), dout.data(), forward, shape);
-  bool plane = false;
 
   BatchCache& cache = batch_cache();
   std::lock_guard<std::mutex> lock(cache.mutex);
@@ -869,75 +973,44 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
   const auto* dArgs = static_cast<const FastArgs<T>*>(plan.args);
   const auto* dMap = static_cast<const int2*>(plan.map);
 
-  if (planeOk) {
-    const int choice = tuned_plane_choice(shape.dimX, forward);
-    if (choice >= 0) {
-      plane = choice != 0;
-    } else {
-      // Time both plane kernels on this batch. The xy launch is idempotent
-      // (backward: [z][stick] -> outputs, after the z launch has filled the
-      // buffer; forward: inputs -> [z][stick]), and the call's real launch
-      // chain follows.
-      if (!forward) {
-        launch_fast_z_batch<T>(false, shape.r2c, shape.dimZ, dArgs, dMap, shape.blocks, nullptr);
-      }
-      hipEvent_t e0 = nullptr, e1 = nullptr;
-      check_hip(hipEventCreate(&e0));
-      check_hip(hipEventCreate(&e1));
-      float ms[2] = {0.f, 0.f};
-      for (int c = 0; c < 2; ++c) {
-        auto xy = [&] {
-          if (c == 1) {
-            launch_plane_xy_batch<T>(forward, shape.dimX, dArgs, nt, shape.dimZ, shape.maxNcol,
-                                     nullptr);
-          } else {
-            launch_fast_xy_batch<T>(forward, shape.r2c, shape.dimX, dArgs, nt, shape.dimZ,
-                                    shape.maxNcol, nullptr);
-          }
-        };
-        xy();
-        check_hip(hipEventRecord(e0, nullptr));
-        for (int k = 0; k < 3; ++k) xy();
-        check_hip(hipEventRecord(e1, nullptr));
-        check_hip(hipEventSynchronize(e1));
-        check_hip(hipEventElapsedTime(&ms[c], e0, e1));
-      }
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
-      plane = ms[1] < ms[0];
-      {
-        TuneTable& tt = tune_table();
-        std::lock_guard<std::mutex> tlock(tt.mutex);
-        tt.choice[std::make_tuple(static_cast<int>(sizeof(T)), shape.dimX, forward ? 1 : 0)] =
-            plane ? 1 : 0;
-      }
-      if (tune_log()) {
-        std::fprintf(stderr, "spfft_amd tune: %s N=%d %s, %zu transforms: class-split %.3f ms, "
-                     "whole-plane %.3f ms -> %s\n", sizeof(T) == 8 ? "double" : "float",
-                     shape.dimX, forward ? "forward" : "backward", n, ms[0] / 3, ms[1] / 3,
-                     plane ? "whole-plane" : "class-split");
-      }
+  // kernel choices: forced plane family, recorded (this process or the cache
+  // file), or timed now on this batch's buffers
+  TuneChoice choice;
+  const int forced = plane_kernels_forced();
+  if (forced >= 0) {
+    choice.plane = forced;
+  } else {
+    const TuneKey tkey = make_tune_key(shape.device, static_cast<int>(sizeof(T)), shape.dimX,
+                                       shape.dimZ, shape.r2c, forward, shape.maxNcol, nt);
+    if (!tune_lookup(tkey, &choice)) {
+      choice = tune_batch(forward, shape, nt, planeOk, dArgs, dMap);
+      tune_record(tkey, choice);
     }
   }
-  if (plan.exec && plan.plane != plane) {
+  if (!planeOk) {
+    choice.plane = 0;
+    choice.slots = 0;
+  }
+  if (plan.exec && (plan.choice.plane != choice.plane || plan.choice.slots != choice.slots ||
+                    plan.choice.zRuns != choice.zRuns)) {
     (void)hipGraphExecDestroy(plan.exec);
     (void)hipGraphDestroy(plan.graph);
     plan.exec = nullptr;
     plan.graph = nullptr;
   }
   if (!graphs_enabled() || prof::enabled()) {
-    launch_batch(forward, shape, nt, plane, dArgs, dMap, nullptr);
+    launch_batch(forward, shape, nt, choice, dArgs, dMap, nullptr);
     return true;
   }
   if (!plan.exec) {
-    plan.plane = plane;
+    plan.choice = choice;
     // Capture the two launches once on a private stream; later calls with
     // the same key replay the graph on the default stream.
     hipStream_t cs = nullptr;
     check_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
     try {
-      launch_batch(forward, shape, nt, plane, dArgs, dMap, cs);
+      launch_batch(forward, shape, nt, choice, dArgs, dMap, cs);
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(cs, &g);
@@ -978,11 +1051,25 @@ void ExecutionGPU<T>::run_batch_noalloc(ExecutionGPU<T>* const* ex, size_t n, co
                     reinterpret_cast<int2*>(image + argBytes));
   // From pageable memory the copy has consumed the stage when it returns.
   check_hip(hipMemcpyAsync(scratch, image, tableBytes, hipMemcpyHostToDevice, nullptr));
-  const bool plane = tuned_plane_choice(shape.dimX, forward) == 1 &&
-                     plane_candidate(ex, n, din, dout, forward, shape);
+  // choices recorded by earlier batched calls of this shape in this process
+  // (a lookup without allocation or file access), else the defaults
+  TuneChoice choice;
+  const int forced = plane_kernels_forced();
+  if (forced >= 0) {
+    choice.plane = forced;
+  } else {
+    (void)tune_recorded(make_tune_key(shape.device, static_cast<int>(sizeof(T)), shape.dimX,
+                                      shape.dimZ, shape.r2c, forward, shape.maxNcol,
+                                      static_cast<int>(n)),
+                        &choice);
+  }
+  if (!plane_candidate(ex, n, din, dout, forward, shape)) {
+    choice.plane = 0;
+    choice.slots = 0;
+  }
   const auto* dArgs = static_cast<const FastArgs<T>*>(scratch);
   const auto* dMap = reinterpret_cast<const int2*>(static_cast<const char*>(scratch) + argBytes);
-  launch_batch(forward, shape, static_cast<int>(n), plane, dArgs, dMap, nullptr);
+  launch_batch(forward, shape, static_cast<int>(n), choice, dArgs, dMap, nullptr);
 }
 
 template class ExecutionGPU<double>;

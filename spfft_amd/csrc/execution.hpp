@@ -16,6 +16,7 @@
 #include "kernels.hpp"
 #include "memory.hpp"
 #include "spfft/types.h"
+#include "tuner.hpp"
 
 namespace spfft {
 
@@ -78,6 +79,8 @@ public:
     int maxNcol = 0;  // widest column set
     bool r2c = false;
     int dimX = 0, dimZ = 0;
+    bool runs = false;  // every transform's sticks are runs in user order
+    int device = 0;
   };
   // True when the n transforms can run as one fused chain; then din/dout (n
   // entries each) hold the device views of the user pointers.
@@ -91,11 +94,15 @@ public:
   static void fill_batch_tables(ExecutionGPU* const* ex, size_t n, const T* const* din,
                                 T* const* dout, bool forward, const SpfftScalingType* scaling,
                                 FastArgs<T>* args, int2* zmap);
-  static void launch_batch(bool forward, const BatchShape& shape, int n, bool plane,
+  // The fused chain with the given kernel choices (choice.plane is honoured
+  // only when the whole-plane kernels can serve the batch).
+  static void launch_batch(bool forward, const BatchShape& shape, int n, const TuneChoice& choice,
                            const FastArgs<T>* dArgs, const int2* dMap, hipStream_t stream);
-  // The plane-kernel family chosen for (precision, length, direction): 1
-  // whole-plane, 0 class-split, -1 not decided yet.
-  static int tuned_plane_choice(int n, bool forward);
+  // Times the candidate kernels on the batch's own buffers and returns the
+  // fastest choice (the launches are idempotent; the call's real chain
+  // follows).
+  static TuneChoice tune_batch(bool forward, const BatchShape& shape, int n, bool planeOk,
+                               const FastArgs<T>* dArgs, const int2* dMap);
 
   // run_batch for callers that cannot allocate (n >= 1 transforms). The
   // argument table and the z block map are written to `scratch`
@@ -143,6 +150,9 @@ private:
                 const hipEvent_t* ready, int readyCount, hipStream_t compute, hipStream_t copy);
   StageArgs<T> base_args() const;
   FastArgs<T> fast_args() const;
+  // Single transforms whose sticks are runs: whether the z kernel addresses
+  // the runs (tuned on the first call of the shape, on that call's buffers).
+  bool z_runs_choice(bool forward, const FastArgs<T>& f, hipStream_t stream);
 
   std::shared_ptr<IndexMap> map_;
   std::shared_ptr<GridInternal<T>> grid_;

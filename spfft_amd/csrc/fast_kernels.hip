@@ -1117,17 +1117,24 @@ template <typename T, int N, bool RC>
 __global__ void __launch_bounds__((XYShape<T, N>::NT)) k_xy_forward_fast(FastArgs<T> a) {
   xy_forward_body<T, N, RC>(a, blockIdx.x);
 }
+// useRuns = 0: the entry-table addressing even for sets made of stick runs
 template <typename T, int NZ, bool RC>
 __global__ void __launch_bounds__((ZShape<T, NZ>::NT))
-    k_z_backward_batch(const FastArgs<T>* __restrict__ batch, const int2* __restrict__ map) {
+    k_z_backward_batch(const FastArgs<T>* __restrict__ batch, const int2* __restrict__ map,
+                       int useRuns) {
   const int2 m = map[blockIdx.x];
-  z_backward_body<T, NZ, RC, true>(on_global(batch[m.x], batch), m.y);
+  FastArgs<T> a = on_global(batch[m.x], batch);
+  a.hasRuns = a.hasRuns && useRuns;
+  z_backward_body<T, NZ, RC, true>(a, m.y);
 }
 template <typename T, int NZ>
 __global__ void __launch_bounds__((ZShape<T, NZ>::NT))
-    k_z_forward_batch(const FastArgs<T>* __restrict__ batch, const int2* __restrict__ map) {
+    k_z_forward_batch(const FastArgs<T>* __restrict__ batch, const int2* __restrict__ map,
+                      int useRuns) {
   const int2 m = map[blockIdx.x];
-  z_forward_body<T, NZ>(on_global(batch[m.x], batch), m.y);
+  FastArgs<T> a = on_global(batch[m.x], batch);
+  a.hasRuns = a.hasRuns && useRuns;
+  z_forward_body<T, NZ>(a, m.y);
 }
 template <typename T, int N, bool RC>
 __global__ void __launch_bounds__((XYShape<T, N>::NT))
@@ -1499,20 +1506,20 @@ void xy_forward_batch_n(bool rc, const FastArgs<T>* batch, int ntrans, int nz,
 
 template <typename T, int NZ>
 void z_batch_n(bool forward, bool rc, const FastArgs<T>* batch, const int2* map, int blocks,
-               hipStream_t stream) {
+               int useRuns, hipStream_t stream) {
   using Z = ZShape<T, NZ>;
   if (forward) {
     auto kernel = k_z_forward_batch<T, NZ>;
     prepare_lds(kernel, Z::lds());
     prof::KernelSpan span(stream, "k_z_forward_batch", true);
     hipExtLaunchKernelGGL(kernel, dim3(blocks), dim3(Z::NT), Z::lds(), stream,
-                          span.start(), span.stop(), 0, batch, map);
+                          span.start(), span.stop(), 0, batch, map, useRuns);
   } else {
     auto kernel = rc ? k_z_backward_batch<T, NZ, true> : k_z_backward_batch<T, NZ, false>;
     prepare_lds(kernel, Z::lds());
     prof::KernelSpan span(stream, rc ? "k_z_backward_batch_r2c" : "k_z_backward_batch", true);
     hipExtLaunchKernelGGL(kernel, dim3(blocks), dim3(Z::NT), Z::lds(), stream,
-                          span.start(), span.stop(), 0, batch, map);
+                          span.start(), span.stop(), 0, batch, map, useRuns);
   }
   check_launch();
 }
@@ -1578,8 +1585,8 @@ int fast_xy_pitch_a(int n, int ncol) {
 
 template <typename T>
 void launch_fast_z_batch(bool forward, bool r2c, int nz, const FastArgs<T>* batch,
-                         const int2* map, int blocks, hipStream_t stream) {
-#define SPFFT_CALL(N) z_batch_n<T, N>(forward, r2c, batch, map, blocks, stream)
+                         const int2* map, int blocks, bool useRuns, hipStream_t stream) {
+#define SPFFT_CALL(N) z_batch_n<T, N>(forward, r2c, batch, map, blocks, useRuns ? 1 : 0, stream)
   SPFFT_FAST_SWITCH(nz, SPFFT_CALL)
 #undef SPFFT_CALL
 }
@@ -1609,9 +1616,26 @@ constexpr bool full_tile_fits() {
   }
 }
 
+// Resident workgroups per CU of the persistent plane kernel (0: not
+// persistent).
+template <typename T, int N>
+int plane_slots_n(bool forward) {
+  if constexpr (full_tile_fits<T, N>()) {
+    using F = FullShape<T, N>;
+    if (!F::PERSISTENT) return 0;
+    auto kernel = forward ? k_full_forward_batch<T, N> : k_full_backward_batch<T, N>;
+    prepare_lds(kernel, F::lds());
+    int perCu = 0;
+    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, F::NT, F::lds()));
+    return std::max(1, perCu);
+  } else {
+    return 0;
+  }
+}
+
 template <typename T, int N>
 void plane_batch_n(bool forward, const FastArgs<T>* batch, int ntrans, int nz, int maxNcol,
-                   hipStream_t stream) {
+                   int slots, hipStream_t stream) {
   if constexpr (full_tile_fits<T, N>()) {
     using F = FullShape<T, N>;
     auto kernel = forward ? k_full_forward_batch<T, N> : k_full_backward_batch<T, N>;
@@ -1624,7 +1648,9 @@ void plane_batch_n(bool forward, const FastArgs<T>* batch, int ntrans, int nz, i
       check_hip(hipGetDevice(&device));
       check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
       check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, F::NT, F::lds()));
-      grid = std::max(1, std::min(total, std::max(1, perCu) * cus));
+      perCu = std::max(1, perCu);
+      if (slots > 0) perCu = std::min(perCu, slots);
+      grid = std::max(1, std::min(total, perCu * cus));
     }
     prof::KernelSpan span(stream, forward ? "k_full_forward_batch" : "k_full_backward_batch",
                           true);
@@ -1674,20 +1700,30 @@ int plane_pitch(int n, int ncol) {
 
 template <typename T>
 void launch_plane_xy_batch(bool forward, int n, const FastArgs<T>* batch, int ntrans, int nz,
-                           int maxNcol, hipStream_t stream) {
-#define SPFFT_CALL(N) plane_batch_n<T, N>(forward, batch, ntrans, nz, maxNcol, stream)
+                           int maxNcol, int slots, hipStream_t stream) {
+#define SPFFT_CALL(N) plane_batch_n<T, N>(forward, batch, ntrans, nz, maxNcol, slots, stream)
   SPFFT_PLANE_SWITCH(n, SPFFT_CALL)
 #undef SPFFT_CALL
+}
+
+template <typename T>
+int plane_xy_slots(bool forward, int n) {
+  int s = 0;
+#define SPFFT_CALL(N) s = plane_slots_n<T, N>(forward)
+  SPFFT_PLANE_SWITCH(n, SPFFT_CALL)
+#undef SPFFT_CALL
+  return s;
 }
 
 #define SPFFT_INSTANTIATE_FAST(T)                                                        \
   template bool plane_path_supported<T>(int, int, bool);                                 \
   template int plane_pitch<T>(int, int);                                                 \
   template void launch_plane_xy_batch<T>(bool, int, const FastArgs<T>*, int, int, int,   \
-                                         hipStream_t);                                   \
+                                         int, hipStream_t);                              \
+  template int plane_xy_slots<T>(bool, int);                                             \
   template int fast_xy_pitch_a<T>(int, int);                                             \
   template void launch_fast_z_batch<T>(bool, bool, int, const FastArgs<T>*, const int2*, \
-                                       int, hipStream_t);                                \
+                                       int, bool, hipStream_t);                          \
   template void launch_fast_xy_batch<T>(bool, bool, int, const FastArgs<T>*, int, int,   \
                                         int, hipStream_t);                               \
   template bool fast_path_supported<T>(int, int, int, int, bool);                        \

@@ -94,9 +94,10 @@ int fast_xy_pitch_a(int n, int ncol);
 // Batched launches over transforms of one shape. `batch` is a device table of
 // arguments; `map` gives (transform, block) for every z workgroup.
 // All transforms of a batch have the same type (r2c).
+// useRuns = false: the per-value entry tables even for sets of stick runs.
 template <typename T>
 void launch_fast_z_batch(bool forward, bool r2c, int nz, const FastArgs<T>* batch,
-                         const int2* map, int blocks, hipStream_t stream);
+                         const int2* map, int blocks, bool useRuns, hipStream_t stream);
 template <typename T>
 void launch_fast_xy_batch(bool forward, bool r2c, int n, const FastArgs<T>* batch, int ntrans,
                           int nz, int maxNcol, hipStream_t stream);
@@ -108,9 +109,15 @@ template <typename T>
 bool plane_path_supported(int n, int ncol, bool r2c);
 template <typename T>
 int plane_pitch(int n, int ncol);
+// slots > 0 caps the resident workgroups per CU of the persistent (FP32
+// full-tile) kernels; 0 keeps all that fit.
 template <typename T>
 void launch_plane_xy_batch(bool forward, int n, const FastArgs<T>* batch, int ntrans, int nz,
-                           int maxNcol, hipStream_t stream);
+                           int maxNcol, int slots, hipStream_t stream);
+// Resident workgroups per CU of the persistent plane kernel for length n (0
+// when that kernel is not persistent).
+template <typename T>
+int plane_xy_slots(bool forward, int n);
 
 template <typename T>
 void launch_fast_z_backward(int nz, const FastArgs<T>& a, hipStream_t stream);
