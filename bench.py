@@ -270,6 +270,50 @@ def launch_floor(torch, kernel_us):
     return out
 
 
+def plain_lengths_bench(args, sp, torch, np):
+    """Single C2C FP64 transforms at lengths without fixed-length kernels
+    (the generic stage kernels serve them), sphere radius 0.3125 N (40 at
+    128): backward+forward pairs/s and the fraction of the HBM roofline of the
+    pair's compulsory bytes, rotating over 8 transforms."""
+    from tests.workloads import complex_values, sphere_triplets
+    out = {}
+    for n in (120, 100, 144):
+        dims = (n, n, n)
+        t = sphere_triplets(0.3125 * n)
+        trs, ins, outs = [], [], []
+        for k in range(args.rotate):
+            tr = sp.Transform(-1, sp.SPFFT_PU_GPU, sp.SPFFT_TRANS_C2C, *dims, len(t),
+                              sp.SPFFT_INDEX_TRIPLETS, t)
+            tr.set_execution_mode(sp.SPFFT_EXEC_ASYNCHRONOUS)
+            ins.append(torch.from_numpy(complex_values(0x5EED0006 + k, len(t)).view(np.float64))
+                       .cuda())
+            outs.append(torch.empty(2 * len(t), dtype=torch.float64, device="cuda"))
+            trs.append(tr)
+
+        def pair(i):
+            k = i % args.rotate
+            trs[k].backward(ins[k].data_ptr(), sp.SPFFT_PU_GPU)
+            trs[k].forward(sp.SPFFT_PU_GPU, outs[k].data_ptr(), sp.SPFFT_FULL_SCALING)
+
+        for i in range(args.warmup):
+            pair(i)
+        torch.cuda.synchronize()
+        steps = max(50, args.steps // 2)
+        t0 = time.perf_counter()
+        for i in range(steps):
+            pair(i)
+        torch.cuda.synchronize()
+        rate = steps / (time.perf_counter() - t0)
+        pair_bytes = 2 * (20 * len(t) + 16 * n ** 3)
+        out[f"{n}^3"] = {"values": len(t), "pairs_per_s": rate,
+                         "frac_hbm_wall": pair_bytes * rate / 1e9 / HBM_PEAK_GBS}
+        log(f"plain C2C {n}^3 r={0.3125 * n:.1f} ({len(t)} values, generic kernels): "
+            f"{rate:.1f} pairs/s, {out[f'{n}^3']['frac_hbm_wall'] * 100:.1f}% of HBM roofline")
+        del trs, ins, outs
+        torch.cuda.empty_cache()
+    return out
+
+
 def plain_r2c_bench(args, sp, torch, np):
     """128^3 R2C FP64, half sphere r=40 (x > 0, or x = 0 and y > 0, or
     x = y = 0 and z >= 0): single transforms, backward+forward pairs/s."""
@@ -540,6 +584,7 @@ def main():
     }
     result["launch_floor"] = launch_floor(torch, d["mean_us"])
     result["plain_r2c"] = plain_r2c_bench(args, sp, torch, np)
+    result["plain_other_lengths"] = plain_lengths_bench(args, sp, torch, np)
     if args.batch > 0:
         b = batched_bench(args, sp, torch, np)
         main_case = b.get("128^3 r=40 x64")

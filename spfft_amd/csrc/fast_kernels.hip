@@ -489,30 +489,41 @@ __device__ __forceinline__ void z_forward_body(const FastArgs<T>& a, int bid) {
 
 // ------------------------------------------------------------ xy kernels
 
+// Full-length lines of the class-split kernels (backward x, forward y): FP64
+// lengths with a factor 3 take 12 elements per lane (radices 4, 4, 2, 3 at
+// 96), since 24 FP64 elements need about 230 registers per lane and halve the
+// waves a SIMD holds.
+template <typename T, int N>
+using FullLine = Line<N, (sizeof(T) == 8 && N % 3 == 0) ? 12 : elems_per_lane(N)>;
+
 template <typename T, int N>
 struct XYShape {
   static constexpr int R = fast_classes(N);     // residue classes per plane
   static constexpr int M = N / R;               // rows (bwd) / columns (fwd) per class
-  static constexpr int NT = N >= 128 && N < 192 ? 512 : 256;
+  using FL = FullLine<T, N>;
+  // enough waves for all rows (columns) of a class in one round of lines
+  static constexpr int NT = (N >= 128 && N < 192) || M > 4 * FL::PER_WAVE ? 512 : 256;
   static constexpr int NW = NT / kWave;
   static_assert(N <= NT, "one twiddle per thread");
-  // lines of M points (backward y, forward x): half the usual elements per
+  // lines of M points (forward x, backward y): half the usual elements per
   // lane, so that a lane's gathers form one batch and twice as many lanes
-  // share them
+  // share them; the backward y lines of FP64 lengths with a factor 3 take a
+  // quarter (6 elements: 12 would keep 48 gathers per lane in flight)
   using YL = Line<M, elems_per_lane(M) / 2>;
+  using YB = Line<M, elems_per_lane(M) / ((sizeof(T) == 8 && M % 3 == 0) ? 4 : 2)>;
   // backward: A = [m][LU] (column u at offset u, y transforms in column
   // mode), reused as B = [rows][LB] (x transforms)
   static constexpr int ROWS_A = padded_len<T>(M);
   static constexpr int LB = padded_len<T>(N);
-  static constexpr int ROWS_ALLOC = round_up(M, Line<N>::PER_WAVE);
-  static_assert(NW * Line<N>::PER_WAVE >= M, "all rows of a class in one round");
+  static constexpr int ROWS_ALLOC = round_up(M, FL::PER_WAVE);
+  static_assert(NW * FL::PER_WAVE >= M, "all rows of a class in one round");
   // forward: Tb = [rows y][LT], columns transformed in place with stride LT;
   // the rows have room for the spare columns of a partial last wave
-  static constexpr int LT = make_odd(padded_len<T>(round_up(M, Line<N>::PER_WAVE)));
+  static constexpr int LT = make_odd(padded_len<T>(round_up(M, FL::PER_WAVE)));
   // Row pitch of A: whole waves of columns including the zero column ncol,
   // and odd, so that the 8 lanes of one column in an FP64 LDS cycle (rows
   // q = 0..7 of the y line) hit distinct bank groups.
-  static int pitch_a(int ncol) { return round_up(ncol + 1, YL::PER_WAVE) | 1; }
+  static int pitch_a(int ncol) { return round_up(ncol + 1, YB::PER_WAVE) | 1; }
   // elements of the A/B region, followed by the twiddle table (N) and, for
   // R2C, the filled x = 0 column (N)
   static size_t tile_backward(int ncol) {
@@ -535,8 +546,8 @@ struct XYShape {
 template <typename T, int N, bool RC>
 __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) {
   using S = XYShape<T, N>;
-  using LM = typename S::YL;
-  using LX = Line<N>;
+  using LM = typename S::YB;
+  using LX = typename S::FL;
   constexpr int R = S::R, M = S::M, LB = S::LB, NW = S::NW;
   const int ncol = a.ncol;
   const int LU = a.pitchA;
@@ -682,7 +693,7 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
   __syncthreads();  // A is dead from here on; its space holds the row lines
   if (wave * LX::PER_WAVE < M) {
     Cx<T>* line = A + m * LB;
-    wave_fft<T, N, true, 1>(v, line, 1, q, twl);
+    wave_fft<T, N, true, 1, LX::E>(v, line, 1, q, twl);
     if (m < M) {
       const size_t row = (static_cast<size_t>(z) * N + r + R * m) * N;
       if (RC) {
@@ -701,7 +712,7 @@ template <typename T, int N, bool RC>
 __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
   using S = XYShape<T, N>;
   using LM = typename S::YL;  // x lines of MX points, half-width like the backward y lines
-  using LY = Line<N>;
+  using LY = typename S::FL;
   constexpr int R = S::R, MX = S::M, LT = S::LT, NW = S::NW;
   Cx<T>* Tb = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
   int z, c;
@@ -773,7 +784,7 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
 #pragma unroll
       for (int e = 0; e < LY::E; ++e) v[e] = col[(q + LY::P * e) * LT];
       wave_sync();
-      wave_fft<T, N, false, 1>(v, col, LT, q, twl);
+      wave_fft<T, N, false, 1, LY::E>(v, col, LT, q, twl);
       if (m < MX) {
 #pragma unroll
         for (int e = 0; e < LY::E; ++e) col[(q + LY::P * e) * LT] = v[e];
