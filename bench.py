@@ -143,7 +143,9 @@ def batched_bench(args, sp, torch, np):
         ratio is taken on this one);
       - 96^3, sphere r=30, 64 bands, FP64 and FP32, values N(0,1) exp(-|G|^2/900);
       - 96^3, 16 k-points x 32 bands, each k-point its own sphere |G+k|^2 <= r^2
-        with k ~ U[-1/2,1/2)^3 and r ~ U[22,30] (uneven sets), FP32.
+        with k ~ U[-1/2,1/2)^3 and r ~ U[22,30] (uneven sets), FP32;
+      - 192^3, sphere r=60, 8 bands, c ~ N(0,1), FP64 and FP32 (the shape of the
+        differentiated workload, whose energy gradient is one such pair).
     Band-pairs/s and the fraction of the HBM roofline (compulsory bytes:
     values + map + dense grid per direction)."""
     from tests.workloads import complex_values, normal, sphere_triplets, uniform, uniform01
@@ -154,6 +156,9 @@ def batched_bench(args, sp, torch, np):
     rs = 22.0 + 8.0 * uniform01(0x5EED0005, 16)
     cases.append(("96^3 16 k-points x32", 96, list(rs), [tuple(k) for k in ks], 32, ("float",),
                   "gauss"))
+    # the differentiated workload's shape: 8 bands of 192^3 r=60, c ~ N(0,1)
+    cases.append(("192^3 r=60 x8", 192, [60.0], [(0.0, 0.0, 0.0)], 8, ("double", "float"),
+                  "normal"))
     for name, n, radii, kpts, bands, precs, dist in cases:
         if args.batch_cases and not any(c in name for c in args.batch_cases):
             continue
@@ -174,6 +179,8 @@ def batched_bench(args, sp, torch, np):
                     seed = 0x5EED0100 + 1000 * si + b
                     if dist == "uniform":
                         v = complex_values(seed, len(t))
+                    elif dist == "normal":
+                        v = normal(seed, len(t)) + 1j * normal(seed + 7919, len(t))
                     else:
                         w = np.exp(-g2 / 900.0)
                         v = (normal(seed, len(t)) + 1j * normal(seed + 7919, len(t))) * w

@@ -507,10 +507,10 @@ struct XYShape {
   static_assert(N <= NT, "one twiddle per thread");
   // lines of M points (forward x, backward y): half the usual elements per
   // lane, so that a lane's gathers form one batch and twice as many lanes
-  // share them; the backward y lines of FP64 lengths with a factor 3 take a
+  // share them; the backward y lines of lengths with a factor 3 take a
   // quarter (6 elements: 12 would keep 48 gathers per lane in flight)
   using YL = Line<M, elems_per_lane(M) / 2>;
-  using YB = Line<M, elems_per_lane(M) / ((sizeof(T) == 8 && M % 3 == 0) ? 4 : 2)>;
+  using YB = Line<M, elems_per_lane(M) / (M % 3 == 0 ? 4 : 2)>;
   // backward: A = [m][LU] (column u at offset u, y transforms in column
   // mode), reused as B = [rows][LB] (x transforms)
   static constexpr int ROWS_A = padded_len<T>(M);
@@ -615,7 +615,8 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
       const int* rel = a.stickAt + static_cast<size_t>(uc) * N;
       const Cx<T>* col = zrow;
       Cx<T> v[LM::E];
-      constexpr int EC = LM::E % 4 == 0 ? 4 : LM::E;  // elements gathered per batch
+      // elements gathered per batch: at most 16 gathers per lane in flight
+      constexpr int EC = (LM::E % 4 == 0 && 4 * R <= 16) ? 4 : (LM::E % 2 == 0 ? 2 : LM::E);
       static_assert(R % 4 == 0, "16-byte index loads");
 #pragma unroll
       for (int e0 = 0; e0 < LM::E; e0 += EC) {
@@ -743,7 +744,8 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
       const Cx<T>* row = in + static_cast<size_t>(yc) * N;
       const T* rrow = a.in + (static_cast<size_t>(z) * N + yc) * N;
       Cx<T> v[LM::E];
-      constexpr int EC = LM::E % 4 == 0 ? 4 : LM::E;
+      // at most 16 loads per lane in flight
+      constexpr int EC = (LM::E % 4 == 0 && 4 * R <= 16) ? 4 : (LM::E % 2 == 0 ? 2 : LM::E);
 #pragma unroll
       for (int e0 = 0; e0 < LM::E; e0 += EC) {
         Cx<T> xv[EC][R];
