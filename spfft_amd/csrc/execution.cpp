@@ -74,14 +74,16 @@ bool disable_fast_path() {
   return v && v[0] == '1';
 }
 
-// Choice of plane kernels for batches: SPFFT_AMD_PLANE=0 keeps the
-// class-split kernels, =1 forces the whole-plane kernels where they apply;
-// unset, the autotuner decides. Returns -1 (tune), 0 or 1. A forced family
-// skips the tuner (defaults for the other choices, nothing recorded).
+// Choice of plane kernels: SPFFT_AMD_PLANE=0 keeps the class-split kernels,
+// =1 forces the whole-plane kernels where they apply (batches), =2 the split
+// plane stage; unset, the autotuner decides. Returns -1 (tune), 0, 1 or 2. A
+// forced family skips the tuner (defaults for the other choices, nothing
+// recorded).
 int plane_kernels_forced() {
   const char* v = std::getenv("SPFFT_AMD_PLANE");
   if (v && v[0] == '0') return 0;
   if (v && v[0] == '1') return 1;
+  if (v && v[0] == '2') return 2;
   return -1;
 }
 
@@ -437,6 +439,8 @@ FastArgs<T> ExecutionGPU<T>::fast_args() const {
   a.hasRuns = tab_->zInfo.data() != nullptr ? 1 : 0;
   a.pitchT = pitchT_;
   a.resYM = tab_->resYM.data();
+  a.ybuf = reinterpret_cast<Cx<T>*>(grid_->work_gpu());
+  a.pitchY = split_pitch<T>(numCols_);
   a.twX = tab_->twX.data();
   a.twY = tab_->twY.data();
   a.twZ = tab_->twZ.data();
@@ -445,39 +449,86 @@ FastArgs<T> ExecutionGPU<T>::fast_args() const {
 }
 
 template <typename T>
-bool ExecutionGPU<T>::z_runs_choice(bool forward, const FastArgs<T>& f, hipStream_t stream) {
-  if (!f.hasRuns) return false;
-  // batched fallbacks on the transform's own stream keep the default
-  if (stream != nullptr) return true;
+TuneChoice ExecutionGPU<T>::single_choice(bool forward, const FastArgs<T>& f, hipStream_t stream,
+                                          bool staged) {
+  TuneChoice c;
+  c.zRuns = f.hasRuns;
+  const int forced = plane_kernels_forced();
+  if (forced >= 0) {
+    c.plane = forced == 2 ? 2 : 0;  // single transforms have no whole-plane kernels
+    return c;
+  }
+  // batched fallbacks on the transform's own stream keep the defaults
+  if (stream != nullptr) return c;
   const TuneKey key = make_tune_key(grid_->device_id(), static_cast<int>(sizeof(T)), map_->dimX,
                                     map_->dimZ, map_->type == SPFFT_TRANS_R2C, forward, numCols_, 1);
-  TuneChoice c;
-  if (tune_lookup(key, &c)) return c.zRuns != 0;
-  // The z launch is idempotent: backward reads the values and writes the
-  // [z][stick] buffer, forward reads that buffer (already written by this
-  // call's plane launches) and writes the values.
+  if (tune_lookup(key, &c)) {
+    c.zRuns = c.zRuns && f.hasRuns;
+    if (c.plane == 1) c.plane = 0;
+    return c;
+  }
+  // Host data staged in the work buffer, which the split stage's candidate
+  // launches would overwrite: no timing on this call (defaults).
+  if (staged) {
+    c = TuneChoice();
+    c.zRuns = f.hasRuns;
+    return c;
+  }
+  // Every launch below is idempotent: backward z reads the values and writes
+  // the [z][stick] buffer, the plane stage reads that buffer and writes the
+  // output; forward the plane stage reads the input and writes the buffer,
+  // z reads it and writes the values. The call's real launches follow.
   TuneEvents ev;
   FastArgs<T> a = f;
-  auto z = [&] {
+  a.zBegin = 0;
+  a.nz = map_->dimZ;
+  auto z = [&](int runs) {
+    a.hasRuns = runs;
     if (forward) {
       launch_fast_z_forward<T>(map_->dimZ, a, nullptr);
     } else {
       launch_fast_z_backward<T>(map_->dimZ, a, nullptr);
     }
   };
-  a.hasRuns = 1;
-  const float runs = time_launches(z, ev.e0, ev.e1);
-  a.hasRuns = 0;
-  const float entries = time_launches(z, ev.e0, ev.e1);
-  c.zRuns = runs <= entries ? 1 : 0;
+  auto xy_fused = [&] {
+    if (forward) {
+      launch_fast_xy_forward<T>(map_->dimX, map_->dimY, a, nullptr);
+    } else {
+      launch_fast_xy_backward<T>(map_->dimX, map_->dimY, a, nullptr);
+    }
+  };
+  auto xy_two = [&] { launch_split_xy<T>(forward, map_->dimX, a, nullptr); };
+  float zr = 0.f, ze = 0.f, fused = 0.f, two = 0.f;
+  auto time_z = [&] {
+    if (f.hasRuns) {
+      zr = time_launches([&] { z(1); }, ev.e0, ev.e1);
+      ze = time_launches([&] { z(0); }, ev.e0, ev.e1);
+      c.zRuns = zr <= ze ? 1 : 0;
+    } else {
+      z(0);
+    }
+  };
+  auto time_xy = [&] {
+    fused = time_launches(xy_fused, ev.e0, ev.e1);
+    two = time_launches(xy_two, ev.e0, ev.e1);
+    c.plane = two < fused ? 2 : 0;
+  };
+  if (forward) {
+    time_xy();
+    time_z();
+  } else {
+    time_z();
+    time_xy();
+  }
   tune_record(key, c);
   if (tune_log_enabled()) {
-    std::fprintf(stderr, "spfft_amd tune: %s N=%d Nz=%d %s %s, single transform: z runs %.4f / "
-                 "entries %.4f ms -> zruns=%d\n", sizeof(T) == 8 ? "double" : "float",
-                 map_->dimX, map_->dimZ, map_->type == SPFFT_TRANS_R2C ? "R2C" : "C2C",
-                 forward ? "forward" : "backward", runs, entries, c.zRuns);
+    std::fprintf(stderr, "spfft_amd tune: %s N=%d Nz=%d %s %s, single transform, %d columns: z "
+                 "runs %.4f / entries %.4f ms; class-split %.4f, split stage %.4f ms -> plane=%d "
+                 "zruns=%d\n", sizeof(T) == 8 ? "double" : "float", map_->dimX, map_->dimZ,
+                 map_->type == SPFFT_TRANS_R2C ? "R2C" : "C2C", forward ? "forward" : "backward",
+                 numCols_, zr, ze, fused, two, c.plane, c.zRuns);
   }
-  return c.zRuns != 0;
+  return c;
 }
 
 template <typename T>
@@ -613,14 +664,20 @@ void ExecutionGPU<T>::backward(const T* input, T* output, bool concurrent) {
     FastArgs<T> f = fast_args();
     f.in = inGPU;
     f.out = outGPU;
-    f.hasRuns = z_runs_choice(false, f, stream) ? 1 : 0;
+    const TuneChoice choice =
+        single_choice(false, f, stream, inKind != MemKind::Device || hostOut);
+    f.hasRuns = choice.zRuns;
     launch_fast_z_backward<T>(map_->dimZ, f, stream);
     // host output: the plane stage runs in chunks of planes, each copied out
     // while the next is computed
     for (int c = 0; c < chunks; ++c) {
       f.zBegin = c * ppc;
       f.nz = std::min(map_->dimZ, (c + 1) * ppc);
-      launch_fast_xy_backward<T>(map_->dimX, map_->dimY, f, stream);
+      if (choice.plane == 2) {
+        launch_split_xy<T>(false, map_->dimX, f, stream);
+      } else {
+        launch_fast_xy_backward<T>(map_->dimX, map_->dimY, f, stream);
+      }
       if (hostOut) check_hip(hipEventRecord(ready[c], stream));
     }
   } else {
@@ -694,16 +751,22 @@ void ExecutionGPU<T>::forward(const T* input, T* output, SpfftScalingType scalin
     f.in = inGPU;
     f.out = outGPU;
     f.useScale = scaling == SPFFT_FULL_SCALING ? 1 : 0;
+    const TuneChoice choice =
+        single_choice(true, f, stream, hostIn || outKind != MemKind::Device);
     // host input: each chunk of planes is transformed as soon as it arrives
     for (int c = 0; c < chunks; ++c) {
       if (hostIn) upload_chunk(c);
       f.zBegin = c * ppc;
       f.nz = std::min(map_->dimZ, (c + 1) * ppc);
-      launch_fast_xy_forward<T>(map_->dimX, map_->dimY, f, stream);
+      if (choice.plane == 2) {
+        launch_split_xy<T>(true, map_->dimX, f, stream);
+      } else {
+        launch_fast_xy_forward<T>(map_->dimX, map_->dimY, f, stream);
+      }
     }
     f.zBegin = 0;
     f.nz = map_->dimZ;
-    f.hasRuns = z_runs_choice(true, f, stream) ? 1 : 0;
+    f.hasRuns = choice.zRuns;
     launch_fast_z_forward<T>(map_->dimZ, f, stream);
   } else {
     if (hostIn) {
@@ -835,7 +898,10 @@ void ExecutionGPU<T>::launch_batch(bool forward, const BatchShape& shape, int n,
                                    const TuneChoice& choice, const FastArgs<T>* dArgs,
                                    const int2* dMap, hipStream_t stream) {
   auto launch_xy = [&] {
-    if (choice.plane) {
+    if (choice.plane == 2) {
+      launch_split_xy_batch<T>(forward, shape.r2c, shape.dimX, dArgs, n, shape.dimZ,
+                               shape.maxNcol, stream);
+    } else if (choice.plane == 1) {
       launch_plane_xy_batch<T>(forward, shape.dimX, dArgs, n, shape.dimZ, shape.maxNcol,
                                choice.slots, stream);
     } else {
@@ -881,11 +947,20 @@ TuneChoice ExecutionGPU<T>::tune_batch(bool forward, const BatchShape& shape, in
   } else if (!forward) {
     z(false);
   }
-  float split = 0.f, plane0 = 0.f, plane1 = 0.f;
+  auto xy_two = [&] {
+    launch_split_xy_batch<T>(forward, shape.r2c, shape.dimX, dArgs, n, shape.dimZ, shape.maxNcol,
+                             nullptr);
+  };
+  float split = 0.f, plane0 = 0.f, plane1 = 0.f, two = 0.f;
+  split = time_launches(xy_split, ev.e0, ev.e1);
+  two = time_launches(xy_two, ev.e0, ev.e1);
+  float best = split;
+  if (two < best) {
+    best = two;
+    c.plane = 2;
+  }
   if (planeOk) {
-    split = time_launches(xy_split, ev.e0, ev.e1);
     plane0 = time_launches([&] { xy_plane(0); }, ev.e0, ev.e1);
-    float best = split;
     if (plane0 < best) {
       best = plane0;
       c.plane = 1;
@@ -903,11 +978,11 @@ TuneChoice ExecutionGPU<T>::tune_batch(bool forward, const BatchShape& shape, in
   if (tune_log_enabled()) {
     std::fprintf(stderr,
                  "spfft_amd tune: %s N=%d Nz=%d %s %s, %d transforms, %d columns: z runs %.3f / "
-                 "entries %.3f ms; class-split %.3f, whole-plane %.3f, one per CU %.3f ms -> "
-                 "plane=%d slots=%d zruns=%d\n",
+                 "entries %.3f ms; class-split %.3f, split stage %.3f, whole-plane %.3f, one per "
+                 "CU %.3f ms -> plane=%d slots=%d zruns=%d\n",
                  sizeof(T) == 8 ? "double" : "float", shape.dimX, shape.dimZ,
                  shape.r2c ? "R2C" : "C2C", forward ? "forward" : "backward", n, shape.maxNcol,
-                 zr, ze, split, plane0, plane1, c.plane, c.slots, c.zRuns);
+                 zr, ze, split, two, plane0, plane1, c.plane, c.slots, c.zRuns);
   }
   return c;
 }
@@ -987,7 +1062,7 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
       tune_record(tkey, choice);
     }
   }
-  if (!planeOk) {
+  if (choice.plane == 1 && !planeOk) {
     choice.plane = 0;
     choice.slots = 0;
   }
@@ -1063,7 +1138,7 @@ void ExecutionGPU<T>::run_batch_noalloc(ExecutionGPU<T>* const* ex, size_t n, co
                                       static_cast<int>(n)),
                         &choice);
   }
-  if (!plane_candidate(ex, n, din, dout, forward, shape)) {
+  if (choice.plane == 1 && !plane_candidate(ex, n, din, dout, forward, shape)) {
     choice.plane = 0;
     choice.slots = 0;
   }

@@ -150,9 +150,11 @@ private:
                 const hipEvent_t* ready, int readyCount, hipStream_t compute, hipStream_t copy);
   StageArgs<T> base_args() const;
   FastArgs<T> fast_args() const;
-  // Single transforms whose sticks are runs: whether the z kernel addresses
-  // the runs (tuned on the first call of the shape, on that call's buffers).
-  bool z_runs_choice(bool forward, const FastArgs<T>& f, hipStream_t stream);
+  // Kernel choices of a single transform (plane stage: class-split or split
+  // kernels; z addressing when the sticks are runs), tuned on the first call
+  // of the shape with device input and output, on that call's buffers
+  // (staged = host data in the work buffer: recorded choices or defaults).
+  TuneChoice single_choice(bool forward, const FastArgs<T>& f, hipStream_t stream, bool staged);
 
   std::shared_ptr<IndexMap> map_;
   std::shared_ptr<GridInternal<T>> grid_;

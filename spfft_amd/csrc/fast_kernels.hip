@@ -990,6 +990,241 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
   }
 }
 
+// ------------------------------------------------------ split plane stage
+//
+// Two kernels per direction around the compact column buffer
+// ybuf[z][y][u] (row pitch pitchY >= ncol, u the sparse set's column):
+//   backward: columns (each line gathers its column's sticks through the
+//             stick index table, N-point y FFT; the workgroup's LINES
+//             columns leave through an LDS tile as row segments), then rows
+//             (each line places the compact row at its x positions, N-point
+//             x FFT, dense row stored);
+//   forward : rows (x FFT, the sparse set's columns stored compact), then
+//             columns (LINES columns enter through the LDS tile as row
+//             segments, y FFT, results written to the sticks).
+// Lines are FullLine<T, N> (P lanes x E elements); LINES = lines per
+// workgroup. No plane is read more than once.
+
+template <typename T, int N>
+struct SplitShape {
+  using L = FullLine<T, N>;
+  // up to 4 waves, never more lines than the plane has rows (columns)
+  static constexpr int NW = N / L::PER_WAVE < 4 ? N / L::PER_WAVE : 4;
+  static constexpr int NT = NW * kWave;
+  static constexpr int LINES = NW * L::PER_WAVE;
+  static_assert(NW >= 1 && LINES <= N, "split line shape");
+  static constexpr int LB = padded_len<T>(N);  // exchange region of one line
+  static constexpr int TP = LINES | 1;         // odd pitch of the [y][LINES] tile
+  // the exchange regions, reused as the tile between phases
+  static constexpr int REGION = LINES * LB > N * TP ? LINES * LB : N * TP;
+  static constexpr int ROW_GROUPS = (N + LINES - 1) / LINES;
+  // region, twiddles, the filled x = 0 column (R2C backward)
+  static constexpr size_t lds() {
+    return static_cast<size_t>(tw_offset<T>(REGION)) * sizeof(Cx<T>) + N * sizeof(Tw<T>) +
+           N * sizeof(Cx<T>);
+  }
+  __host__ __device__ static int col_groups(int ncol) { return (ncol + LINES - 1) / LINES; }
+};
+
+// Stick index of (column u, y) in the class-interleaved table of the xy
+// kernels (y = j + M l at (u M + j) R + l).
+template <int N>
+__device__ __forceinline__ int stick_at(const int* table, int u, int y) {
+  constexpr int R = fast_classes(N), M = N / R;
+  return table[(u * M + y % M) * R + y / M];
+}
+
+template <typename T, int N, bool RC>
+__device__ __forceinline__ void ysplit_backward_body(const FastArgs<T>& a, int bid) {
+  using S = SplitShape<T, N>;
+  using L = typename S::L;
+  constexpr int E = L::E, P = L::P;
+  const int groups = S::col_groups(a.ncol);
+  const int z = bid / groups + a.zBegin;
+  const int u0 = (bid % groups) * S::LINES;
+  if (z >= a.nz) return;
+  Cx<T>* region = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
+  Tw<T>* twl = reinterpret_cast<Tw<T>*>(region + tw_offset<T>(S::REGION));
+  Cx<T>* fill = reinterpret_cast<Cx<T>*>(twl + N);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int q = lane % P;
+  const int line = wave * L::PER_WAVE + lane / P;
+  const int u = u0 + line;
+  const int uc = u < a.ncol ? u : a.ncol;  // row ncol of the table: the zero element
+  const Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
+  Cx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = at32(zrow, stick_at<N>(a.stickAt, uc, q + P * e));
+  for (int t = threadIdx.x; t < N; t += S::NT) twl[t] = a.twY[t];
+  if (RC && u0 == 0 && a.colX[0] == 0) {
+    // hermitian fill of the x = 0 column (reference symmetry_host.hpp:47-58)
+    if (line == 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) fill[q + P * e] = v[e];
+    }
+    __syncthreads();
+    for (int k = 1 + threadIdx.x; 2 * k <= N; k += S::NT) {
+      Cx<T>& lo = fill[k];
+      Cx<T>& hi = fill[N - k];
+      if (2 * k == N) {
+        if (cnonzero(lo)) lo = cconj(lo);
+      } else if (cnonzero(lo)) {
+        hi = cconj(lo);
+      } else if (cnonzero(hi)) {
+        lo = cconj(hi);
+      }
+    }
+    __syncthreads();
+    if (line == 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = fill[q + P * e];
+    }
+  } else {
+    __syncthreads();  // twiddles
+  }
+  Cx<T>* lr = region + line * S::LB;
+  wave_fft<T, N, true, 1, E>(v, lr, 1, q, twl);
+  __syncthreads();  // the exchange regions become the tile
+#pragma unroll
+  for (int e = 0; e < E; ++e) region[(q + P * e) * S::TP + line] = v[e];
+  __syncthreads();
+  Cx<T>* yrows = a.ybuf + static_cast<size_t>(z) * N * a.pitchY + u0;
+  const int nc = min(S::LINES, a.ncol - u0);
+  for (int t = threadIdx.x; t < N * S::LINES; t += S::NT) {
+    const int y = t / S::LINES, c = t - y * S::LINES;
+    if (c < nc) yrows[static_cast<size_t>(y) * a.pitchY + c] = region[y * S::TP + c];
+  }
+}
+
+template <typename T, int N, bool RC>
+__device__ __forceinline__ void xsplit_backward_body(const FastArgs<T>& a, int bid) {
+  using S = SplitShape<T, N>;
+  using L = typename S::L;
+  constexpr int E = L::E, P = L::P;
+  const int z = bid / S::ROW_GROUPS + a.zBegin;
+  const int y0 = (bid % S::ROW_GROUPS) * S::LINES;
+  if (z >= a.nz) return;
+  Cx<T>* region = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
+  Tw<T>* twl = reinterpret_cast<Tw<T>*>(region + tw_offset<T>(S::REGION));
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int q = lane % P;
+  const int line = wave * L::PER_WAVE + lane / P;
+  const int y = y0 + line;
+  const int yc = y < N ? y : N - 1;
+  const int ncol = a.ncol;
+  const Cx<T>* yrow = a.ybuf + (static_cast<size_t>(z) * N + yc) * a.pitchY;
+  // compact row at its x positions; absent x read column 0 and are zeroed;
+  // R2C: the upper half is the conjugate of x' = N - x
+  Cx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int x = q + P * e;
+    const int u = a.xCol[(RC && 2 * x > N) ? N - x : x];
+    const Cx<T> c = yrow[u < ncol ? u : 0];
+    Cx<T> w = u < ncol ? c : Cx<T>{T(0), T(0)};
+    if (RC) {
+      if (x == 0 || 2 * x == N) w.im = T(0);
+      if (2 * x > N) w = cconj(w);
+    }
+    v[e] = w;
+  }
+  for (int t = threadIdx.x; t < N; t += S::NT) twl[t] = a.twX[t];
+  __syncthreads();
+  wave_fft<T, N, true, 1, E>(v, region + line * S::LB, 1, q, twl);
+  if (y < N) {
+    const size_t row = (static_cast<size_t>(z) * N + y) * N;
+    if (RC) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) a.out[row + q + P * e] = v[e].re;
+    } else {
+      Cx<T>* out = reinterpret_cast<Cx<T>*>(a.out) + row;
+#pragma unroll
+      for (int e = 0; e < E; ++e) out[q + P * e] = v[e];
+    }
+  }
+}
+
+template <typename T, int N, bool RC>
+__device__ __forceinline__ void xsplit_forward_body(const FastArgs<T>& a, int bid) {
+  using S = SplitShape<T, N>;
+  using L = typename S::L;
+  constexpr int E = L::E, P = L::P;
+  const int z = bid / S::ROW_GROUPS + a.zBegin;
+  const int y0 = (bid % S::ROW_GROUPS) * S::LINES;
+  if (z >= a.nz) return;
+  Cx<T>* region = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
+  Tw<T>* twl = reinterpret_cast<Tw<T>*>(region + tw_offset<T>(S::REGION));
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int q = lane % P;
+  const int line = wave * L::PER_WAVE + lane / P;
+  const int y = y0 + line;
+  const int yc = y < N ? y : N - 1;
+  const size_t row = (static_cast<size_t>(z) * N + yc) * N;
+  Cx<T> v[E];
+  if (RC) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = Cx<T>{a.in[row + q + P * e], T(0)};
+  } else {
+    const Cx<T>* in = reinterpret_cast<const Cx<T>*>(a.in) + row;
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = in[q + P * e];
+  }
+  int uu[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) uu[e] = a.xCol[q + P * e];
+  for (int t = threadIdx.x; t < N; t += S::NT) twl[t] = a.twX[t];
+  __syncthreads();
+  wave_fft<T, N, false, 1, E>(v, region + line * S::LB, 1, q, twl);
+  if (y < N) {
+    Cx<T>* yrow = a.ybuf + (static_cast<size_t>(z) * N + y) * a.pitchY;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (uu[e] < a.ncol) yrow[uu[e]] = v[e];
+    }
+  }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void ysplit_forward_body(const FastArgs<T>& a, int bid) {
+  using S = SplitShape<T, N>;
+  using L = typename S::L;
+  constexpr int E = L::E, P = L::P;
+  const int groups = S::col_groups(a.ncol);
+  const int z = bid / groups + a.zBegin;
+  const int u0 = (bid % groups) * S::LINES;
+  if (z >= a.nz) return;
+  Cx<T>* region = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
+  Tw<T>* twl = reinterpret_cast<Tw<T>*>(region + tw_offset<T>(S::REGION));
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int q = lane % P;
+  const int line = wave * L::PER_WAVE + lane / P;
+  const int u = u0 + line;
+  const int nc = min(S::LINES, a.ncol - u0);
+  // LINES columns of every row through the tile (row segments); columns past
+  // ncol are zero
+  const Cx<T>* yrows = a.ybuf + static_cast<size_t>(z) * N * a.pitchY + u0;
+  for (int t = threadIdx.x; t < N * S::LINES; t += S::NT) {
+    const int y = t / S::LINES, c = t - y * S::LINES;
+    region[y * S::TP + c] =
+        c < nc ? yrows[static_cast<size_t>(y) * a.pitchY + c] : Cx<T>{T(0), T(0)};
+  }
+  for (int t = threadIdx.x; t < N; t += S::NT) twl[t] = a.twY[t];
+  __syncthreads();
+  Cx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = region[(q + P * e) * S::TP + line];
+  __syncthreads();  // the tile becomes the exchange regions
+  wave_fft<T, N, false, 1, E>(v, region + line * S::LB, 1, q, twl);
+  if (u < a.ncol) {
+    Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int s = stick_at<N>(a.stickAt, u, q + P * e);
+      if (s < a.nst) zrow[s] = v[e];
+    }
+  }
+}
+
 // ------------------------------------------------- full-tile plane kernels
 //
 // One workgroup per plane with the whole plane in LDS: tile row y holds the
@@ -1103,6 +1338,7 @@ __device__ __forceinline__ FastArgs<T> on_global(FastArgs<T> a, const void* tabl
   a.resSticks = rebase(g, a.resSticks);
   a.resBegin = rebase(g, a.resBegin);
   a.resYM = rebase(g, a.resYM);
+  a.ybuf = rebase(g, a.ybuf);
   a.twX = rebase(g, a.twX);
   a.twY = rebase(g, a.twY);
   a.twZ = rebase(g, a.twZ);
@@ -1160,6 +1396,47 @@ __global__ void __launch_bounds__((XYShape<T, N>::NT))
     k_xy_forward_batch(const FastArgs<T>* __restrict__ batch, int perTransform) {
   const int t = blockIdx.x / perTransform;
   xy_forward_body<T, N, RC>(on_global(batch[t], batch), blockIdx.x - t * perTransform);
+}
+
+template <typename T, int N, bool RC>
+__global__ void __launch_bounds__((SplitShape<T, N>::NT)) k_ysplit_backward(FastArgs<T> a) {
+  ysplit_backward_body<T, N, RC>(a, blockIdx.x);
+}
+template <typename T, int N, bool RC>
+__global__ void __launch_bounds__((SplitShape<T, N>::NT)) k_xsplit_backward(FastArgs<T> a) {
+  xsplit_backward_body<T, N, RC>(a, blockIdx.x);
+}
+template <typename T, int N, bool RC>
+__global__ void __launch_bounds__((SplitShape<T, N>::NT)) k_xsplit_forward(FastArgs<T> a) {
+  xsplit_forward_body<T, N, RC>(a, blockIdx.x);
+}
+template <typename T, int N>
+__global__ void __launch_bounds__((SplitShape<T, N>::NT)) k_ysplit_forward(FastArgs<T> a) {
+  ysplit_forward_body<T, N>(a, blockIdx.x);
+}
+template <typename T, int N, bool RC>
+__global__ void __launch_bounds__((SplitShape<T, N>::NT))
+    k_ysplit_backward_batch(const FastArgs<T>* __restrict__ batch, int perTransform) {
+  const int t = blockIdx.x / perTransform;
+  ysplit_backward_body<T, N, RC>(on_global(batch[t], batch), blockIdx.x - t * perTransform);
+}
+template <typename T, int N, bool RC>
+__global__ void __launch_bounds__((SplitShape<T, N>::NT))
+    k_xsplit_backward_batch(const FastArgs<T>* __restrict__ batch, int perTransform) {
+  const int t = blockIdx.x / perTransform;
+  xsplit_backward_body<T, N, RC>(on_global(batch[t], batch), blockIdx.x - t * perTransform);
+}
+template <typename T, int N, bool RC>
+__global__ void __launch_bounds__((SplitShape<T, N>::NT))
+    k_xsplit_forward_batch(const FastArgs<T>* __restrict__ batch, int perTransform) {
+  const int t = blockIdx.x / perTransform;
+  xsplit_forward_body<T, N, RC>(on_global(batch[t], batch), blockIdx.x - t * perTransform);
+}
+template <typename T, int N>
+__global__ void __launch_bounds__((SplitShape<T, N>::NT))
+    k_ysplit_forward_batch(const FastArgs<T>* __restrict__ batch, int perTransform) {
+  const int t = blockIdx.x / perTransform;
+  ysplit_forward_body<T, N>(on_global(batch[t], batch), blockIdx.x - t * perTransform);
 }
 
 template <typename T, int N>
@@ -1728,12 +2005,86 @@ int plane_xy_slots(bool forward, int n) {
   return s;
 }
 
+// ---- split plane stage launches
+
+template <typename T, int N, typename Kernel, typename... Args>
+void split_launch(Kernel kernel, const char* name, int blocks, hipStream_t stream, Args... args) {
+  using S = SplitShape<T, N>;
+  prepare_lds(kernel, S::lds());
+  prof::KernelSpan span(stream, name, true);
+  hipExtLaunchKernelGGL(kernel, dim3(blocks), dim3(S::NT), S::lds(), stream, span.start(),
+                        span.stop(), 0, args...);
+  check_launch();
+}
+
+template <typename T, int N>
+void split_n(bool forward, const FastArgs<T>& a, hipStream_t stream) {
+  using S = SplitShape<T, N>;
+  const int planes = a.nz - a.zBegin;
+  const int cb = S::col_groups(a.ncol) * planes, rb = S::ROW_GROUPS * planes;
+  if (forward) {
+    split_launch<T, N>(a.r2c ? k_xsplit_forward<T, N, true> : k_xsplit_forward<T, N, false>,
+                       "k_xsplit_forward", rb, stream, a);
+    split_launch<T, N>(k_ysplit_forward<T, N>, "k_ysplit_forward", cb, stream, a);
+  } else {
+    split_launch<T, N>(a.r2c ? k_ysplit_backward<T, N, true> : k_ysplit_backward<T, N, false>,
+                       "k_ysplit_backward", cb, stream, a);
+    split_launch<T, N>(a.r2c ? k_xsplit_backward<T, N, true> : k_xsplit_backward<T, N, false>,
+                       "k_xsplit_backward", rb, stream, a);
+  }
+}
+
+template <typename T, int N>
+void split_batch_n(bool forward, bool rc, const FastArgs<T>* batch, int ntrans, int nz,
+                   int maxNcol, hipStream_t stream) {
+  using S = SplitShape<T, N>;
+  const int cper = S::col_groups(maxNcol) * nz, rper = S::ROW_GROUPS * nz;
+  if (forward) {
+    split_launch<T, N>(rc ? k_xsplit_forward_batch<T, N, true> : k_xsplit_forward_batch<T, N, false>,
+                       "k_xsplit_forward_batch", rper * ntrans, stream, batch, rper);
+    split_launch<T, N>(k_ysplit_forward_batch<T, N>, "k_ysplit_forward_batch", cper * ntrans,
+                       stream, batch, cper);
+  } else {
+    split_launch<T, N>(rc ? k_ysplit_backward_batch<T, N, true>
+                          : k_ysplit_backward_batch<T, N, false>,
+                       "k_ysplit_backward_batch", cper * ntrans, stream, batch, cper);
+    split_launch<T, N>(rc ? k_xsplit_backward_batch<T, N, true>
+                          : k_xsplit_backward_batch<T, N, false>,
+                       "k_xsplit_backward_batch", rper * ntrans, stream, batch, rper);
+  }
+}
+
+template <typename T>
+int split_pitch(int ncol) {
+  const int q = static_cast<int>(128 / sizeof(Cx<T>));  // 128-byte row segments
+  return (ncol + q - 1) / q * q;
+}
+
+template <typename T>
+void launch_split_xy(bool forward, int n, const FastArgs<T>& a, hipStream_t stream) {
+#define SPFFT_CALL(N) split_n<T, N>(forward, a, stream)
+  SPFFT_FAST_SWITCH(n, SPFFT_CALL)
+#undef SPFFT_CALL
+}
+
+template <typename T>
+void launch_split_xy_batch(bool forward, bool r2c, int n, const FastArgs<T>* batch, int ntrans,
+                           int nz, int maxNcol, hipStream_t stream) {
+#define SPFFT_CALL(N) split_batch_n<T, N>(forward, r2c, batch, ntrans, nz, maxNcol, stream)
+  SPFFT_FAST_SWITCH(n, SPFFT_CALL)
+#undef SPFFT_CALL
+}
+
 #define SPFFT_INSTANTIATE_FAST(T)                                                        \
   template bool plane_path_supported<T>(int, int, bool);                                 \
   template int plane_pitch<T>(int, int);                                                 \
   template void launch_plane_xy_batch<T>(bool, int, const FastArgs<T>*, int, int, int,   \
                                          int, hipStream_t);                              \
   template int plane_xy_slots<T>(bool, int);                                             \
+  template int split_pitch<T>(int);                                                      \
+  template void launch_split_xy<T>(bool, int, const FastArgs<T>&, hipStream_t);          \
+  template void launch_split_xy_batch<T>(bool, bool, int, const FastArgs<T>*, int, int,  \
+                                         int, hipStream_t);                              \
   template int fast_xy_pitch_a<T>(int, int);                                             \
   template void launch_fast_z_batch<T>(bool, bool, int, const FastArgs<T>*, const int2*, \
                                        int, bool, hipStream_t);                          \

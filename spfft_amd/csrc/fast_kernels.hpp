@@ -60,6 +60,8 @@ struct FastArgs {
   const int* resBegin = nullptr;      // R + 1 offsets into resSticks
   const int* resYM = nullptr;         // per resSticks entry: y << 16 | (x / R)
   int pitchT = 0;                     // whole-plane kernels: row pitch of the plane tile
+  Cx<T>* ybuf = nullptr;              // split plane kernels: compact columns [z][y][pitchY]
+  int pitchY = 0;                     //   (column u of row y at u; pitchY >= ncol)
   const Tw<T>* twX = nullptr;
   const Tw<T>* twY = nullptr;
   const Tw<T>* twZ = nullptr;
@@ -118,6 +120,20 @@ void launch_plane_xy_batch(bool forward, int n, const FastArgs<T>* batch, int nt
 // when that kernel is not persistent).
 template <typename T>
 int plane_xy_slots(bool forward, int n);
+
+// Split plane stage: two kernels per direction with the compact column
+// buffer ybuf between them (y stage on the sparse set's columns, x stage on
+// rows). Every plane element is read once from memory, at the cost of the
+// intermediate; it serves large lengths better than the class-split kernels,
+// which read each plane once per residue class. Single transforms (planes
+// [zBegin, nz)) and batches.
+template <typename T>
+int split_pitch(int ncol);
+template <typename T>
+void launch_split_xy(bool forward, int n, const FastArgs<T>& a, hipStream_t stream);
+template <typename T>
+void launch_split_xy_batch(bool forward, bool r2c, int n, const FastArgs<T>* batch, int ntrans,
+                           int nz, int maxNcol, hipStream_t stream);
 
 template <typename T>
 void launch_fast_z_backward(int nz, const FastArgs<T>& a, hipStream_t stream);

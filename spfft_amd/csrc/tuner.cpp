@@ -78,7 +78,7 @@ void load_locked(TuneState& s) {
     TuneChoice c;
     if (std::sscanf(line, "%255s plane=%d slots=%d zruns=%d", key, &c.plane, &c.slots,
                     &c.zRuns) == 4 &&
-        (c.plane == 0 || c.plane == 1) && c.slots >= 0 && c.slots <= 64 &&
+        c.plane >= 0 && c.plane <= 2 && c.slots >= 0 && c.slots <= 64 &&
         (c.zRuns == 0 || c.zRuns == 1)) {
       s.choices[key] = c;
     }

@@ -31,7 +31,8 @@ struct TuneKey {
 };
 
 struct TuneChoice {
-  int plane = 0;  // batches: 1 whole-plane kernels, 0 class-split kernels
+  int plane = 0;  // plane stage: 0 class-split kernels, 1 whole-plane kernels (batches),
+                  // 2 split stage (y and x kernels around a compact column buffer)
   int slots = 0;  // persistent plane kernels: resident workgroups per CU (0: all that fit)
   int zRuns = 1;  // z kernels address stick runs (when every stick of the set is a run)
 };

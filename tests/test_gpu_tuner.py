@@ -108,3 +108,45 @@ def test_tuner_cache_off(tmp_path):
     log, _ = run_child(tmp_path, "off", "off")
     assert "spfft_amd tune:" in log
     assert not any(p.name.startswith("tune") for p in tmp_path.rglob("*"))
+
+
+HOST_FIRST = r'''
+import sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import spfft_amd as sp
+from oracle import oracle as orc
+from tests.workloads import complex_values, sphere_triplets
+# The first call of each shape takes pageable host buffers (staged through the
+# grid's work buffer): the choices must not be timed on that buffer.
+for n, prec in ((32, "float"), (64, "double"), (96, "float"), (192, "double")):
+    dims = (n, n, n)
+    t = sphere_triplets(0.3 * n)
+    v = complex_values(0x5EED0A20 + n, len(t))
+    cls = sp.Transform if prec == "double" else sp.TransformFloat
+    rd = np.float64 if prec == "double" else np.float32
+    tol = 1e-12 if prec == "double" else 1e-5
+    tr = cls(-1, sp.SPFFT_PU_GPU, sp.SPFFT_TRANS_C2C, *dims, len(t), sp.SPFFT_INDEX_TRIPLETS, t)
+    space = np.zeros(2 * n ** 3, rd)
+    tr.backward(np.ascontiguousarray(v.view(np.float64).astype(rd)), space)
+    o = orc.OraclePlan(False, dims, t)
+    want = o.backward(v)
+    assert orc.rel_l2(space.astype(np.float64).view(np.complex128).reshape(n, n, n), want) <= tol
+    out = np.zeros(2 * len(t), rd)
+    tr.forward(np.ascontiguousarray(want.view(np.float64).ravel().astype(rd)), out,
+               sp.SPFFT_FULL_SCALING)
+    assert orc.rel_l2(out.astype(np.float64).view(np.complex128), o.forward(want, True)) <= tol
+print("child done", flush=True)
+'''
+
+
+def test_first_call_with_host_pointers(tmp_path):
+    env = dict(os.environ)
+    env["SPFFT_AMD_TUNE_CACHE"] = "off"
+    env.pop("SPFFT_AMD_PLANE", None)
+    script = tmp_path / "host_first.py"
+    script.write_text(HOST_FIRST)
+    print("tuner child host_first", flush=True)
+    r = subprocess.run([sys.executable, "-u", str(script), ROOT], env=env, capture_output=True,
+                       text=True, timeout=240, cwd=ROOT)
+    assert r.returncode == 0, r.stdout + r.stderr
