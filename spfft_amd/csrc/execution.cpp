@@ -194,7 +194,7 @@ std::shared_ptr<TransformTables<T>> build_tables(const std::shared_ptr<IndexMap>
     // kernel reads all the indices of one decimation-in-frequency output with
     // R / 4 16-byte loads.
     const int ny = map->dimY;
-    const int R = fast_classes(map->dimX), M = ny / R;
+    const int R = fast_classes(ny), M = ny / R;
     std::vector<int> at(static_cast<size_t>(tb.numCols + 1) * ny, map->num_sticks());
     for (int s = 0; s < map->num_sticks(); ++s) {
       const int y = t.stickY[s];
@@ -271,7 +271,8 @@ std::shared_ptr<TransformTables<T>> build_tables(const std::shared_ptr<IndexMap>
     tb.resSticks = DeviceBuffer<int>::from_host(resSticks);
     tb.resBegin = DeviceBuffer<int>::from_host(resBegin);
     // tile pitch for the whole-plane batch kernels (0: not applicable)
-    if (plane_path_supported<T>(map->dimX, tb.numCols, map->type == SPFFT_TRANS_R2C)) {
+    if (map->dimX == map->dimY &&
+        plane_path_supported<T>(map->dimX, tb.numCols, map->type == SPFFT_TRANS_R2C)) {
       tb.pitchT = plane_pitch<T>(map->dimX, tb.numCols);
     }
   }
@@ -417,6 +418,7 @@ template <typename T>
 FastArgs<T> ExecutionGPU<T>::fast_args() const {
   FastArgs<T> a;
   a.nz = map_->dimZ;
+  a.ny = map_->dimY;
   a.nst = map_->num_sticks();
   a.ncol = numCols_;
   a.zbuf = reinterpret_cast<Cx<T>*>(grid_->sticks_gpu());
@@ -453,18 +455,23 @@ TuneChoice ExecutionGPU<T>::single_choice(bool forward, const FastArgs<T>& f, hi
                                           bool staged) {
   TuneChoice c;
   c.zRuns = f.hasRuns;
+  const bool square = map_->dimX == map_->dimY;
   const int forced = plane_kernels_forced();
-  if (forced >= 0) {
-    c.plane = forced == 2 ? 2 : 0;  // single transforms have no whole-plane kernels
-    return c;
+  if (forced >= 0 || !square) {
+    // single transforms have no whole-plane kernels; x != y planes only the
+    // split stage
+    c.plane = forced == 2 || !square ? 2 : 0;
+    if (forced >= 0) return c;
   }
   // batched fallbacks on the transform's own stream keep the defaults
   if (stream != nullptr) return c;
   const TuneKey key = make_tune_key(grid_->device_id(), static_cast<int>(sizeof(T)), map_->dimX,
-                                    map_->dimZ, map_->type == SPFFT_TRANS_R2C, forward, numCols_, 1);
+                                    map_->dimY, map_->dimZ, map_->type == SPFFT_TRANS_R2C,
+                                    forward, numCols_, 1);
   if (tune_lookup(key, &c)) {
     c.zRuns = c.zRuns && f.hasRuns;
     if (c.plane == 1) c.plane = 0;
+    if (!square) c.plane = 2;
     return c;
   }
   // Host data staged in the work buffer, which the split stage's candidate
@@ -472,6 +479,7 @@ TuneChoice ExecutionGPU<T>::single_choice(bool forward, const FastArgs<T>& f, hi
   if (staged) {
     c = TuneChoice();
     c.zRuns = f.hasRuns;
+    c.plane = square ? 0 : 2;
     return c;
   }
   // Every launch below is idempotent: backward z reads the values and writes
@@ -497,7 +505,7 @@ TuneChoice ExecutionGPU<T>::single_choice(bool forward, const FastArgs<T>& f, hi
       launch_fast_xy_backward<T>(map_->dimX, map_->dimY, a, nullptr);
     }
   };
-  auto xy_two = [&] { launch_split_xy<T>(forward, map_->dimX, a, nullptr); };
+  auto xy_two = [&] { launch_split_xy<T>(forward, map_->dimX, map_->dimY, a, nullptr); };
   float zr = 0.f, ze = 0.f, fused = 0.f, two = 0.f;
   auto time_z = [&] {
     if (f.hasRuns) {
@@ -509,9 +517,13 @@ TuneChoice ExecutionGPU<T>::single_choice(bool forward, const FastArgs<T>& f, hi
     }
   };
   auto time_xy = [&] {
-    fused = time_launches(xy_fused, ev.e0, ev.e1);
     two = time_launches(xy_two, ev.e0, ev.e1);
-    c.plane = two < fused ? 2 : 0;
+    if (square) {
+      fused = time_launches(xy_fused, ev.e0, ev.e1);
+      c.plane = two < fused ? 2 : 0;
+    } else {
+      c.plane = 2;
+    }
   };
   if (forward) {
     time_xy();
@@ -674,7 +686,7 @@ void ExecutionGPU<T>::backward(const T* input, T* output, bool concurrent) {
       f.zBegin = c * ppc;
       f.nz = std::min(map_->dimZ, (c + 1) * ppc);
       if (choice.plane == 2) {
-        launch_split_xy<T>(false, map_->dimX, f, stream);
+        launch_split_xy<T>(false, map_->dimX, map_->dimY, f, stream);
       } else {
         launch_fast_xy_backward<T>(map_->dimX, map_->dimY, f, stream);
       }
@@ -759,7 +771,7 @@ void ExecutionGPU<T>::forward(const T* input, T* output, SpfftScalingType scalin
       f.zBegin = c * ppc;
       f.nz = std::min(map_->dimZ, (c + 1) * ppc);
       if (choice.plane == 2) {
-        launch_split_xy<T>(true, map_->dimX, f, stream);
+        launch_split_xy<T>(true, map_->dimX, map_->dimY, f, stream);
       } else {
         launch_fast_xy_forward<T>(map_->dimX, map_->dimY, f, stream);
       }
@@ -850,6 +862,7 @@ bool ExecutionGPU<T>::batch_qualifies(ExecutionGPU<T>* const* ex, size_t n, cons
   }
   shape->r2c = m0.type == SPFFT_TRANS_R2C;
   shape->dimX = m0.dimX;
+  shape->dimY = m0.dimY;
   shape->dimZ = m0.dimZ;
   shape->device = ex[0]->grid_->device_id();
   return true;
@@ -860,7 +873,7 @@ bool ExecutionGPU<T>::plane_candidate(ExecutionGPU<T>* const* ex, size_t n, cons
                                       T* const* dout, bool forward, const BatchShape& shape) {
   // whole-plane kernels are a candidate when every transform has its tile
   // positions and the widest tile fits
-  bool ok = plane_kernels_forced() != 0 &&
+  bool ok = plane_kernels_forced() != 0 && shape.dimX == shape.dimY &&
             plane_path_supported<T>(shape.dimX, shape.maxNcol, shape.r2c);
   for (size_t t = 0; t < n && ok; ++t) ok = ex[t]->pitchT_ > 0;
   // The FP32 whole-plane kernels move two elements per 16-byte access, so
@@ -899,7 +912,7 @@ void ExecutionGPU<T>::launch_batch(bool forward, const BatchShape& shape, int n,
                                    const int2* dMap, hipStream_t stream) {
   auto launch_xy = [&] {
     if (choice.plane == 2) {
-      launch_split_xy_batch<T>(forward, shape.r2c, shape.dimX, dArgs, n, shape.dimZ,
+      launch_split_xy_batch<T>(forward, shape.r2c, shape.dimX, shape.dimY, dArgs, n, shape.dimZ,
                                shape.maxNcol, stream);
     } else if (choice.plane == 1) {
       launch_plane_xy_batch<T>(forward, shape.dimX, dArgs, n, shape.dimZ, shape.maxNcol,
@@ -936,9 +949,19 @@ TuneChoice ExecutionGPU<T>::tune_batch(bool forward, const BatchShape& shape, in
     launch_plane_xy_batch<T>(forward, shape.dimX, dArgs, n, shape.dimZ, shape.maxNcol, slots,
                              nullptr);
   };
+  auto xy_two = [&] {
+    launch_split_xy_batch<T>(forward, shape.r2c, shape.dimX, shape.dimY, dArgs, n, shape.dimZ,
+                             shape.maxNcol, nullptr);
+  };
   // The forward z kernel reads the [z][stick] buffer that the xy kernel
   // writes, the backward xy kernel the one that the z kernel writes.
-  if (forward) xy_split();
+  if (forward) {
+    if (shape.dimX == shape.dimY) {
+      xy_split();
+    } else {
+      xy_two();
+    }
+  }
   float zr = 0.f, ze = 0.f;
   if (shape.runs) {
     zr = time_launches([&] { z(true); }, ev.e0, ev.e1);
@@ -947,17 +970,17 @@ TuneChoice ExecutionGPU<T>::tune_batch(bool forward, const BatchShape& shape, in
   } else if (!forward) {
     z(false);
   }
-  auto xy_two = [&] {
-    launch_split_xy_batch<T>(forward, shape.r2c, shape.dimX, dArgs, n, shape.dimZ, shape.maxNcol,
-                             nullptr);
-  };
   float split = 0.f, plane0 = 0.f, plane1 = 0.f, two = 0.f;
-  split = time_launches(xy_split, ev.e0, ev.e1);
   two = time_launches(xy_two, ev.e0, ev.e1);
-  float best = split;
-  if (two < best) {
-    best = two;
-    c.plane = 2;
+  float best = two;
+  c.plane = 2;
+  if (shape.dimX == shape.dimY) {
+    // x != y planes have the split stage only
+    split = time_launches(xy_split, ev.e0, ev.e1);
+    if (split <= best) {
+      best = split;
+      c.plane = 0;
+    }
   }
   if (planeOk) {
     plane0 = time_launches([&] { xy_plane(0); }, ev.e0, ev.e1);
@@ -1053,10 +1076,11 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
   TuneChoice choice;
   const int forced = plane_kernels_forced();
   if (forced >= 0) {
-    choice.plane = forced;
+    choice.plane = shape.dimX == shape.dimY ? forced : 2;
   } else {
     const TuneKey tkey = make_tune_key(shape.device, static_cast<int>(sizeof(T)), shape.dimX,
-                                       shape.dimZ, shape.r2c, forward, shape.maxNcol, nt);
+                                       shape.dimY, shape.dimZ, shape.r2c, forward,
+                                       shape.maxNcol, nt);
     if (!tune_lookup(tkey, &choice)) {
       choice = tune_batch(forward, shape, nt, planeOk, dArgs, dMap);
       tune_record(tkey, choice);
@@ -1066,6 +1090,7 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
     choice.plane = 0;
     choice.slots = 0;
   }
+  if (shape.dimX != shape.dimY) choice.plane = 2;
   if (plan.exec && (plan.choice.plane != choice.plane || plan.choice.slots != choice.slots ||
                     plan.choice.zRuns != choice.zRuns)) {
     (void)hipGraphExecDestroy(plan.exec);
@@ -1134,10 +1159,11 @@ void ExecutionGPU<T>::run_batch_noalloc(ExecutionGPU<T>* const* ex, size_t n, co
     choice.plane = forced;
   } else {
     (void)tune_recorded(make_tune_key(shape.device, static_cast<int>(sizeof(T)), shape.dimX,
-                                      shape.dimZ, shape.r2c, forward, shape.maxNcol,
+                                      shape.dimY, shape.dimZ, shape.r2c, forward, shape.maxNcol,
                                       static_cast<int>(n)),
                         &choice);
   }
+  if (shape.dimX != shape.dimY) choice.plane = 2;
   if (choice.plane == 1 && !plane_candidate(ex, n, din, dout, forward, shape)) {
     choice.plane = 0;
     choice.slots = 0;

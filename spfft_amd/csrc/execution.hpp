@@ -78,7 +78,7 @@ public:
     int blocks = 0;   // z workgroups of the whole batch
     int maxNcol = 0;  // widest column set
     bool r2c = false;
-    int dimX = 0, dimZ = 0;
+    int dimX = 0, dimY = 0, dimZ = 0;
     bool runs = false;  // every transform's sticks are runs in user order
     int device = 0;
   };

@@ -1017,7 +1017,8 @@ struct SplitShape {
   static constexpr int TP = LINES | 1;         // odd pitch of the [y][LINES] tile
   // the exchange regions, reused as the tile between phases
   static constexpr int REGION = LINES * LB > N * TP ? LINES * LB : N * TP;
-  static constexpr int ROW_GROUPS = (N + LINES - 1) / LINES;
+  // workgroups per plane of the row kernels (rows of length N, ny rows)
+  __host__ __device__ static int row_groups(int ny) { return (ny + LINES - 1) / LINES; }
   // region, twiddles, the filled x = 0 column (R2C backward)
   static constexpr size_t lds() {
     return static_cast<size_t>(tw_offset<T>(REGION)) * sizeof(Cx<T>) + N * sizeof(Tw<T>) +
@@ -1101,8 +1102,10 @@ __device__ __forceinline__ void xsplit_backward_body(const FastArgs<T>& a, int b
   using S = SplitShape<T, N>;
   using L = typename S::L;
   constexpr int E = L::E, P = L::P;
-  const int z = bid / S::ROW_GROUPS + a.zBegin;
-  const int y0 = (bid % S::ROW_GROUPS) * S::LINES;
+  const int ny = a.ny;  // rows of the plane (N is the row length)
+  const int groups = S::row_groups(ny);
+  const int z = bid / groups + a.zBegin;
+  const int y0 = (bid % groups) * S::LINES;
   if (z >= a.nz) return;
   Cx<T>* region = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
   Tw<T>* twl = reinterpret_cast<Tw<T>*>(region + tw_offset<T>(S::REGION));
@@ -1110,9 +1113,9 @@ __device__ __forceinline__ void xsplit_backward_body(const FastArgs<T>& a, int b
   const int q = lane % P;
   const int line = wave * L::PER_WAVE + lane / P;
   const int y = y0 + line;
-  const int yc = y < N ? y : N - 1;
+  const int yc = y < ny ? y : ny - 1;
   const int ncol = a.ncol;
-  const Cx<T>* yrow = a.ybuf + (static_cast<size_t>(z) * N + yc) * a.pitchY;
+  const Cx<T>* yrow = a.ybuf + (static_cast<size_t>(z) * ny + yc) * a.pitchY;
   // compact row at its x positions; absent x read column 0 and are zeroed;
   // R2C: the upper half is the conjugate of x' = N - x
   Cx<T> v[E];
@@ -1131,8 +1134,8 @@ __device__ __forceinline__ void xsplit_backward_body(const FastArgs<T>& a, int b
   for (int t = threadIdx.x; t < N; t += S::NT) twl[t] = a.twX[t];
   __syncthreads();
   wave_fft<T, N, true, 1, E>(v, region + line * S::LB, 1, q, twl);
-  if (y < N) {
-    const size_t row = (static_cast<size_t>(z) * N + y) * N;
+  if (y < ny) {
+    const size_t row = (static_cast<size_t>(z) * ny + y) * N;
     if (RC) {
 #pragma unroll
       for (int e = 0; e < E; ++e) a.out[row + q + P * e] = v[e].re;
@@ -1149,8 +1152,10 @@ __device__ __forceinline__ void xsplit_forward_body(const FastArgs<T>& a, int bi
   using S = SplitShape<T, N>;
   using L = typename S::L;
   constexpr int E = L::E, P = L::P;
-  const int z = bid / S::ROW_GROUPS + a.zBegin;
-  const int y0 = (bid % S::ROW_GROUPS) * S::LINES;
+  const int ny = a.ny;  // rows of the plane (N is the row length)
+  const int groups = S::row_groups(ny);
+  const int z = bid / groups + a.zBegin;
+  const int y0 = (bid % groups) * S::LINES;
   if (z >= a.nz) return;
   Cx<T>* region = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
   Tw<T>* twl = reinterpret_cast<Tw<T>*>(region + tw_offset<T>(S::REGION));
@@ -1158,8 +1163,8 @@ __device__ __forceinline__ void xsplit_forward_body(const FastArgs<T>& a, int bi
   const int q = lane % P;
   const int line = wave * L::PER_WAVE + lane / P;
   const int y = y0 + line;
-  const int yc = y < N ? y : N - 1;
-  const size_t row = (static_cast<size_t>(z) * N + yc) * N;
+  const int yc = y < ny ? y : ny - 1;
+  const size_t row = (static_cast<size_t>(z) * ny + yc) * N;
   Cx<T> v[E];
   if (RC) {
 #pragma unroll
@@ -1175,8 +1180,8 @@ __device__ __forceinline__ void xsplit_forward_body(const FastArgs<T>& a, int bi
   for (int t = threadIdx.x; t < N; t += S::NT) twl[t] = a.twX[t];
   __syncthreads();
   wave_fft<T, N, false, 1, E>(v, region + line * S::LB, 1, q, twl);
-  if (y < N) {
-    Cx<T>* yrow = a.ybuf + (static_cast<size_t>(z) * N + y) * a.pitchY;
+  if (y < ny) {
+    Cx<T>* yrow = a.ybuf + (static_cast<size_t>(z) * ny + y) * a.pitchY;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       if (uu[e] < a.ncol) yrow[uu[e]] = v[e];
@@ -1826,7 +1831,9 @@ void fast_z_geometry(int nz, int* sticksPerBlock, int* lineLength, int* padShift
 
 template <typename T>
 bool fast_path_supported(int nx, int ny, int nz, int ncol, bool r2c) {
-  if (nx != ny || !fast_length(nx) || !fast_length(nz) || ncol < 1) return false;
+  if (!fast_length(nx) || !fast_length(ny) || !fast_length(nz) || ncol < 1) return false;
+  // x != y: the split plane stage (y and x kernels of their own lengths)
+  if (nx != ny) return true;
   bool ok = false;
 #define SPFFT_FITS(N) ok = fits<T, N>(ncol, r2c)
   SPFFT_FAST_SWITCH(nx, SPFFT_FITS)
@@ -2017,40 +2024,58 @@ void split_launch(Kernel kernel, const char* name, int blocks, hipStream_t strea
   check_launch();
 }
 
-template <typename T, int N>
-void split_n(bool forward, const FastArgs<T>& a, hipStream_t stream) {
-  using S = SplitShape<T, N>;
-  const int planes = a.nz - a.zBegin;
-  const int cb = S::col_groups(a.ncol) * planes, rb = S::ROW_GROUPS * planes;
+// One stage of the split plane stage: the y kernel (columns of length NY)
+// or the x kernel (rows of length NX, ny rows).
+template <typename T, int NY>
+void ysplit_n(bool forward, const FastArgs<T>& a, hipStream_t stream) {
+  using S = SplitShape<T, NY>;
+  const int blocks = S::col_groups(a.ncol) * (a.nz - a.zBegin);
   if (forward) {
-    split_launch<T, N>(a.r2c ? k_xsplit_forward<T, N, true> : k_xsplit_forward<T, N, false>,
-                       "k_xsplit_forward", rb, stream, a);
-    split_launch<T, N>(k_ysplit_forward<T, N>, "k_ysplit_forward", cb, stream, a);
+    split_launch<T, NY>(k_ysplit_forward<T, NY>, "k_ysplit_forward", blocks, stream, a);
   } else {
-    split_launch<T, N>(a.r2c ? k_ysplit_backward<T, N, true> : k_ysplit_backward<T, N, false>,
-                       "k_ysplit_backward", cb, stream, a);
-    split_launch<T, N>(a.r2c ? k_xsplit_backward<T, N, true> : k_xsplit_backward<T, N, false>,
-                       "k_xsplit_backward", rb, stream, a);
+    split_launch<T, NY>(a.r2c ? k_ysplit_backward<T, NY, true> : k_ysplit_backward<T, NY, false>,
+                        "k_ysplit_backward", blocks, stream, a);
   }
 }
-
-template <typename T, int N>
-void split_batch_n(bool forward, bool rc, const FastArgs<T>* batch, int ntrans, int nz,
-                   int maxNcol, hipStream_t stream) {
-  using S = SplitShape<T, N>;
-  const int cper = S::col_groups(maxNcol) * nz, rper = S::ROW_GROUPS * nz;
+template <typename T, int NX>
+void xsplit_n(bool forward, const FastArgs<T>& a, hipStream_t stream) {
+  using S = SplitShape<T, NX>;
+  const int blocks = S::row_groups(a.ny) * (a.nz - a.zBegin);
   if (forward) {
-    split_launch<T, N>(rc ? k_xsplit_forward_batch<T, N, true> : k_xsplit_forward_batch<T, N, false>,
-                       "k_xsplit_forward_batch", rper * ntrans, stream, batch, rper);
-    split_launch<T, N>(k_ysplit_forward_batch<T, N>, "k_ysplit_forward_batch", cper * ntrans,
-                       stream, batch, cper);
+    split_launch<T, NX>(a.r2c ? k_xsplit_forward<T, NX, true> : k_xsplit_forward<T, NX, false>,
+                        "k_xsplit_forward", blocks, stream, a);
   } else {
-    split_launch<T, N>(rc ? k_ysplit_backward_batch<T, N, true>
-                          : k_ysplit_backward_batch<T, N, false>,
-                       "k_ysplit_backward_batch", cper * ntrans, stream, batch, cper);
-    split_launch<T, N>(rc ? k_xsplit_backward_batch<T, N, true>
-                          : k_xsplit_backward_batch<T, N, false>,
-                       "k_xsplit_backward_batch", rper * ntrans, stream, batch, rper);
+    split_launch<T, NX>(a.r2c ? k_xsplit_backward<T, NX, true> : k_xsplit_backward<T, NX, false>,
+                        "k_xsplit_backward", blocks, stream, a);
+  }
+}
+template <typename T, int NY>
+void ysplit_batch_n(bool forward, bool rc, const FastArgs<T>* batch, int ntrans, int nz,
+                    int maxNcol, hipStream_t stream) {
+  using S = SplitShape<T, NY>;
+  const int per = S::col_groups(maxNcol) * nz;
+  if (forward) {
+    split_launch<T, NY>(k_ysplit_forward_batch<T, NY>, "k_ysplit_forward_batch", per * ntrans,
+                        stream, batch, per);
+  } else {
+    split_launch<T, NY>(rc ? k_ysplit_backward_batch<T, NY, true>
+                           : k_ysplit_backward_batch<T, NY, false>,
+                        "k_ysplit_backward_batch", per * ntrans, stream, batch, per);
+  }
+}
+template <typename T, int NX>
+void xsplit_batch_n(bool forward, bool rc, const FastArgs<T>* batch, int ntrans, int ny, int nz,
+                    hipStream_t stream) {
+  using S = SplitShape<T, NX>;
+  const int per = S::row_groups(ny) * nz;
+  if (forward) {
+    split_launch<T, NX>(rc ? k_xsplit_forward_batch<T, NX, true>
+                           : k_xsplit_forward_batch<T, NX, false>,
+                        "k_xsplit_forward_batch", per * ntrans, stream, batch, per);
+  } else {
+    split_launch<T, NX>(rc ? k_xsplit_backward_batch<T, NX, true>
+                           : k_xsplit_backward_batch<T, NX, false>,
+                        "k_xsplit_backward_batch", per * ntrans, stream, batch, per);
   }
 }
 
@@ -2061,18 +2086,35 @@ int split_pitch(int ncol) {
 }
 
 template <typename T>
-void launch_split_xy(bool forward, int n, const FastArgs<T>& a, hipStream_t stream) {
-#define SPFFT_CALL(N) split_n<T, N>(forward, a, stream)
-  SPFFT_FAST_SWITCH(n, SPFFT_CALL)
+void launch_split_xy(bool forward, int nx, int ny, const FastArgs<T>& a, hipStream_t stream) {
+  // backward: y then x; forward: x then y
+  for (int k = 0; k < 2; ++k) {
+    if ((k == 0) != forward) {
+#define SPFFT_CALL(N) ysplit_n<T, N>(forward, a, stream)
+      SPFFT_FAST_SWITCH(ny, SPFFT_CALL)
 #undef SPFFT_CALL
+    } else {
+#define SPFFT_CALL(N) xsplit_n<T, N>(forward, a, stream)
+      SPFFT_FAST_SWITCH(nx, SPFFT_CALL)
+#undef SPFFT_CALL
+    }
+  }
 }
 
 template <typename T>
-void launch_split_xy_batch(bool forward, bool r2c, int n, const FastArgs<T>* batch, int ntrans,
-                           int nz, int maxNcol, hipStream_t stream) {
-#define SPFFT_CALL(N) split_batch_n<T, N>(forward, r2c, batch, ntrans, nz, maxNcol, stream)
-  SPFFT_FAST_SWITCH(n, SPFFT_CALL)
+void launch_split_xy_batch(bool forward, bool r2c, int nx, int ny, const FastArgs<T>* batch,
+                           int ntrans, int nz, int maxNcol, hipStream_t stream) {
+  for (int k = 0; k < 2; ++k) {
+    if ((k == 0) != forward) {
+#define SPFFT_CALL(N) ysplit_batch_n<T, N>(forward, r2c, batch, ntrans, nz, maxNcol, stream)
+      SPFFT_FAST_SWITCH(ny, SPFFT_CALL)
 #undef SPFFT_CALL
+    } else {
+#define SPFFT_CALL(N) xsplit_batch_n<T, N>(forward, r2c, batch, ntrans, ny, nz, stream)
+      SPFFT_FAST_SWITCH(nx, SPFFT_CALL)
+#undef SPFFT_CALL
+    }
+  }
 }
 
 #define SPFFT_INSTANTIATE_FAST(T)                                                        \
@@ -2082,9 +2124,9 @@ void launch_split_xy_batch(bool forward, bool r2c, int n, const FastArgs<T>* bat
                                          int, hipStream_t);                              \
   template int plane_xy_slots<T>(bool, int);                                             \
   template int split_pitch<T>(int);                                                      \
-  template void launch_split_xy<T>(bool, int, const FastArgs<T>&, hipStream_t);          \
-  template void launch_split_xy_batch<T>(bool, bool, int, const FastArgs<T>*, int, int,  \
-                                         int, hipStream_t);                              \
+  template void launch_split_xy<T>(bool, int, int, const FastArgs<T>&, hipStream_t);     \
+  template void launch_split_xy_batch<T>(bool, bool, int, int, const FastArgs<T>*, int,  \
+                                         int, int, hipStream_t);                         \
   template int fast_xy_pitch_a<T>(int, int);                                             \
   template void launch_fast_z_batch<T>(bool, bool, int, const FastArgs<T>*, const int2*, \
                                        int, bool, hipStream_t);                          \

@@ -31,6 +31,7 @@ namespace spfft {
 template <typename T>
 struct FastArgs {
   int nz = 0, nst = 0, ncol = 0;
+  int ny = 0;      // rows of a plane (the split stage's x kernels)
   int zBegin = 0;  // single-transform xy launches: planes [zBegin, nz) (chunked host I/O)
   int pitch = 0;   // row pitch of zbuf in elements (nst + 1 rounded up to 128 bytes);
                    // element nst of every row is zero
@@ -85,7 +86,8 @@ inline int fast_pitch(int nst) {
   return (nst + 1 + q - 1) / q * q;
 }
 
-// True when (nx, ny, nz) have fast kernels and their LDS fits for ncol columns.
+// True when (nx, ny, nz) have fast kernels and their LDS fits for ncol columns
+// (nx != ny: the split plane stage only).
 template <typename T>
 bool fast_path_supported(int nx, int ny, int nz, int ncol, bool r2c);
 
@@ -130,10 +132,10 @@ int plane_xy_slots(bool forward, int n);
 template <typename T>
 int split_pitch(int ncol);
 template <typename T>
-void launch_split_xy(bool forward, int n, const FastArgs<T>& a, hipStream_t stream);
+void launch_split_xy(bool forward, int nx, int ny, const FastArgs<T>& a, hipStream_t stream);
 template <typename T>
-void launch_split_xy_batch(bool forward, bool r2c, int n, const FastArgs<T>* batch, int ntrans,
-                           int nz, int maxNcol, hipStream_t stream);
+void launch_split_xy_batch(bool forward, bool r2c, int nx, int ny, const FastArgs<T>* batch,
+                           int ntrans, int nz, int maxNcol, hipStream_t stream);
 
 template <typename T>
 void launch_fast_z_backward(int nz, const FastArgs<T>& a, hipStream_t stream);

@@ -24,11 +24,11 @@ struct TuneState {
   std::map<std::string, TuneChoice> choices;
   std::map<int, std::string> arch;  // device -> "gfx950:256"
   // choices this process has used, by numeric key
-  std::map<std::tuple<int, int, int, int, int, int, int, int>, TuneChoice> used;
+  std::map<std::tuple<int, int, int, int, int, int, int, int, int>, TuneChoice> used;
 };
 
-std::tuple<int, int, int, int, int, int, int, int> numeric(const TuneKey& k) {
-  return std::make_tuple(k.device, k.valueBytes, k.n, k.nz, k.r2c, k.forward, k.ncolBucket,
+std::tuple<int, int, int, int, int, int, int, int, int> numeric(const TuneKey& k) {
+  return std::make_tuple(k.device, k.valueBytes, k.n, k.ny, k.nz, k.r2c, k.forward, k.ncolBucket,
                          k.countBucket);
 }
 
@@ -92,7 +92,7 @@ void save_locked(const TuneState& s) {
   const std::string tmp = s.path + ".tmp" + std::to_string(static_cast<long>(getpid()));
   FILE* f = std::fopen(tmp.c_str(), "w");
   if (!f) return;
-  std::fprintf(f, "# spfft_amd kernel choices: <arch:CUs:bytes:N:Nz:type:direction:"
+  std::fprintf(f, "# spfft_amd kernel choices: <arch:CUs:bytes:N[xNy]:Nz:type:direction:"
                   "columns:transforms> plane= slots= zruns=\n");
   for (const auto& kv : s.choices) {
     std::fprintf(f, "%s plane=%d slots=%d zruns=%d\n", kv.first.c_str(), kv.second.plane,
@@ -110,12 +110,13 @@ int round_pow2(int v) {
 
 }  // namespace
 
-TuneKey make_tune_key(int device, int valueBytes, int n, int nz, bool r2c, bool forward,
+TuneKey make_tune_key(int device, int valueBytes, int n, int ny, int nz, bool r2c, bool forward,
                       int maxNcol, int count) {
   TuneKey k;
   k.device = device;
   k.valueBytes = valueBytes;
   k.n = n;
+  k.ny = ny;
   k.nz = nz;
   k.r2c = r2c ? 1 : 0;
   k.forward = forward ? 1 : 0;
@@ -147,7 +148,9 @@ std::string tune_key_text(const TuneKey& k) {
     s.arch[k.device] = arch;
   }
   std::ostringstream o;
-  o << arch << ":f" << 8 * k.valueBytes << ":N" << k.n << ":Z" << k.nz << ":"
+  o << arch << ":f" << 8 * k.valueBytes << ":N" << k.n;
+  if (k.ny != k.n) o << "x" << k.ny;
+  o << ":Z" << k.nz << ":"
     << (k.r2c ? "r2c" : "c2c") << ":" << (k.forward ? "fwd" : "bwd") << ":col" << k.ncolBucket
     << ":x" << k.countBucket;
   return o.str();

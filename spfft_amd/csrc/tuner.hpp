@@ -2,8 +2,7 @@
 // of a shape (on that call's own buffers), kept per process and persisted in a
 // per-shape cache file, so later processes start from the measured choices.
 //
-// Key: device architecture and CU count, value bytes, Nx (= Ny), Nz, C2C or
-// R2C, direction, widest column count (rounded up to 16) and transform count
+// Key: device architecture and CU count, value bytes, Nx, Ny, Nz, C2C or R2C, direction, widest column count (rounded up to 16) and transform count
 // (rounded up to a power of two; 1 for single transforms).
 // Choices: the plane-kernel family of batches (class-split kernels with R
 // residue classes per plane and M-point decimation-in-frequency lines, or
@@ -25,7 +24,7 @@ namespace spfft {
 struct TuneKey {
   int device = 0;
   int valueBytes = 0;
-  int n = 0, nz = 0;
+  int n = 0, ny = 0, nz = 0;
   int r2c = 0, forward = 0;
   int ncolBucket = 0, countBucket = 0;
 };
@@ -37,7 +36,7 @@ struct TuneChoice {
   int zRuns = 1;  // z kernels address stick runs (when every stick of the set is a run)
 };
 
-TuneKey make_tune_key(int device, int valueBytes, int n, int nz, bool r2c, bool forward,
+TuneKey make_tune_key(int device, int valueBytes, int n, int ny, int nz, bool r2c, bool forward,
                       int maxNcol, int count);
 // The recorded choice for `key` (from this process or the cache file).
 bool tune_lookup(const TuneKey& key, TuneChoice* choice);
