@@ -176,6 +176,8 @@ std::shared_ptr<TransformTables<T>> build_tables(const std::shared_ptr<IndexMap>
   // fixed-length kernels when the lengths have compile-time schedules
   tb.fast = fast && fast_path_supported<T>(map->dimX, map->dimY, map->dimZ, tb.numCols,
                                            map->type == SPFFT_TRANS_R2C);
+  tb.classSplit = tb.fast && class_split_supported<T>(map->dimX, map->dimY, tb.numCols,
+                                                      map->type == SPFFT_TRANS_R2C);
   const bool useFast = tb.fast;
   tb.entryValue = DeviceBuffer<int>::from_host(t.entryValue);
   tb.entrySlot = DeviceBuffer<int>::from_host(t.entrySlot);
@@ -351,6 +353,7 @@ ExecutionGPU<T>::ExecutionGPU(const std::shared_ptr<IndexMap>& map,
   tab_ = acquire_tables<T>(map_, fast, grid_->device_id(), stream_);
   numCols_ = tab_->numCols;
   fast_ = tab_->fast;
+  classSplit_ = tab_->classSplit;
   pitchT_ = tab_->pitchT;
 
   fftX_ = make_fft_desc(map_->dimX);
@@ -455,12 +458,12 @@ TuneChoice ExecutionGPU<T>::single_choice(bool forward, const FastArgs<T>& f, hi
                                           bool staged) {
   TuneChoice c;
   c.zRuns = f.hasRuns;
-  const bool square = map_->dimX == map_->dimY;
+  const bool classSplit = classSplit_;
   const int forced = plane_kernels_forced();
-  if (forced >= 0 || !square) {
-    // single transforms have no whole-plane kernels; x != y planes only the
-    // split stage
-    c.plane = forced == 2 || !square ? 2 : 0;
+  if (forced >= 0 || !classSplit) {
+    // single transforms have no whole-plane kernels; x != y planes and
+    // lengths without class-split kernels only the split stage
+    c.plane = forced == 2 || !classSplit ? 2 : 0;
     if (forced >= 0) return c;
   }
   // batched fallbacks on the transform's own stream keep the defaults
@@ -471,7 +474,7 @@ TuneChoice ExecutionGPU<T>::single_choice(bool forward, const FastArgs<T>& f, hi
   if (tune_lookup(key, &c)) {
     c.zRuns = c.zRuns && f.hasRuns;
     if (c.plane == 1) c.plane = 0;
-    if (!square) c.plane = 2;
+    if (!classSplit) c.plane = 2;
     return c;
   }
   // Host data staged in the work buffer, which the split stage's candidate
@@ -479,7 +482,7 @@ TuneChoice ExecutionGPU<T>::single_choice(bool forward, const FastArgs<T>& f, hi
   if (staged) {
     c = TuneChoice();
     c.zRuns = f.hasRuns;
-    c.plane = square ? 0 : 2;
+    c.plane = classSplit ? 0 : 2;
     return c;
   }
   // Every launch below is idempotent: backward z reads the values and writes
@@ -518,7 +521,7 @@ TuneChoice ExecutionGPU<T>::single_choice(bool forward, const FastArgs<T>& f, hi
   };
   auto time_xy = [&] {
     two = time_launches(xy_two, ev.e0, ev.e1);
-    if (square) {
+    if (classSplit) {
       fused = time_launches(xy_fused, ev.e0, ev.e1);
       c.plane = two < fused ? 2 : 0;
     } else {
@@ -855,7 +858,9 @@ bool ExecutionGPU<T>::batch_qualifies(ExecutionGPU<T>* const* ex, size_t n, cons
   shape->blocks = 0;
   shape->maxNcol = 0;
   shape->runs = true;
+  shape->classSplit = true;
   for (size_t t = 0; t < n; ++t) {
+    shape->classSplit = shape->classSplit && ex[t]->classSplit_;
     shape->blocks += (ex[t]->map_->num_sticks() + S - 1) / S;
     shape->maxNcol = std::max(shape->maxNcol, ex[t]->numCols_);
     shape->runs = shape->runs && ex[t]->tab_->zInfo.data() != nullptr;
@@ -899,7 +904,7 @@ void ExecutionGPU<T>::fill_batch_tables(ExecutionGPU<T>* const* ex, size_t n, co
     f.in = din[t];
     f.out = dout[t];
     f.useScale = forward && scaling[t] == SPFFT_FULL_SCALING ? 1 : 0;
-    f.pitchA = fast_xy_pitch_a<T>(ex[0]->map_->dimX, f.ncol);
+    f.pitchA = ex[t]->classSplit_ ? fast_xy_pitch_a<T>(ex[0]->map_->dimX, f.ncol) : 0;
     const int b = (f.nst + S - 1) / S;
     for (int j = 0; j < b; ++j) zmap[k++] = make_int2(static_cast<int>(t), j);
     args[t] = f;
@@ -956,7 +961,7 @@ TuneChoice ExecutionGPU<T>::tune_batch(bool forward, const BatchShape& shape, in
   // The forward z kernel reads the [z][stick] buffer that the xy kernel
   // writes, the backward xy kernel the one that the z kernel writes.
   if (forward) {
-    if (shape.dimX == shape.dimY) {
+    if (shape.classSplit) {
       xy_split();
     } else {
       xy_two();
@@ -974,8 +979,9 @@ TuneChoice ExecutionGPU<T>::tune_batch(bool forward, const BatchShape& shape, in
   two = time_launches(xy_two, ev.e0, ev.e1);
   float best = two;
   c.plane = 2;
-  if (shape.dimX == shape.dimY) {
-    // x != y planes have the split stage only
+  if (shape.classSplit) {
+    // x != y planes and lengths without class-split kernels have the split
+    // stage only
     split = time_launches(xy_split, ev.e0, ev.e1);
     if (split <= best) {
       best = split;
@@ -1076,7 +1082,7 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
   TuneChoice choice;
   const int forced = plane_kernels_forced();
   if (forced >= 0) {
-    choice.plane = shape.dimX == shape.dimY ? forced : 2;
+    choice.plane = forced;
   } else {
     const TuneKey tkey = make_tune_key(shape.device, static_cast<int>(sizeof(T)), shape.dimX,
                                        shape.dimY, shape.dimZ, shape.r2c, forward,
@@ -1090,7 +1096,7 @@ bool ExecutionGPU<T>::run_batch(const std::vector<ExecutionGPU<T>*>& ex,
     choice.plane = 0;
     choice.slots = 0;
   }
-  if (shape.dimX != shape.dimY) choice.plane = 2;
+  if (choice.plane == 0 && !shape.classSplit) choice.plane = 2;
   if (plan.exec && (plan.choice.plane != choice.plane || plan.choice.slots != choice.slots ||
                     plan.choice.zRuns != choice.zRuns)) {
     (void)hipGraphExecDestroy(plan.exec);
@@ -1163,11 +1169,11 @@ void ExecutionGPU<T>::run_batch_noalloc(ExecutionGPU<T>* const* ex, size_t n, co
                                       static_cast<int>(n)),
                         &choice);
   }
-  if (shape.dimX != shape.dimY) choice.plane = 2;
   if (choice.plane == 1 && !plane_candidate(ex, n, din, dout, forward, shape)) {
     choice.plane = 0;
     choice.slots = 0;
   }
+  if (choice.plane == 0 && !shape.classSplit) choice.plane = 2;
   const auto* dArgs = static_cast<const FastArgs<T>*>(scratch);
   const auto* dMap = reinterpret_cast<const int2*>(static_cast<const char*>(scratch) + argBytes);
   launch_batch(forward, shape, static_cast<int>(n), choice, dArgs, dMap, nullptr);

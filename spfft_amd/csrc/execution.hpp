@@ -30,6 +30,7 @@ struct TransformTables {
   int device = 0;
   bool allowFast = true;  // built with the fixed-length path allowed
   bool fast = false;      // fixed-length kernels serve this set
+  bool classSplit = false;  // and the class-split plane kernels (nx = ny, LDS fits)
   int numCols = 0;
   int pitchT = 0;  // whole-plane batch kernels' tile pitch (0: not applicable)
   DeviceBuffer<Tw<T>> twX, twY, twZ;
@@ -80,6 +81,7 @@ public:
     bool r2c = false;
     int dimX = 0, dimY = 0, dimZ = 0;
     bool runs = false;  // every transform's sticks are runs in user order
+    bool classSplit = false;  // class-split plane kernels serve every transform
     int device = 0;
   };
   // True when the n transforms can run as one fused chain; then din/dout (n
@@ -168,6 +170,7 @@ private:
   std::shared_ptr<TransformTables<T>> tab_;  // shared with transforms of the same set
   // fixed-length path (two kernels per direction)
   bool fast_ = false;
+  bool classSplit_ = false;  // class-split plane kernels too (else the split stage only)
   // whole-plane batch kernels (power-of-two lengths up to 128, C2C): tile pitch
   int pitchT_ = 0;
   // host I/O (created on the first call with a host pointer): chunk-ready

@@ -57,6 +57,47 @@ constexpr int wave_radix(int m, int e) {
          : (m % 5 == 0 && e % 5 == 0) ? 5
                                       : -1;
 }
+// True when every pass of the wave-level schedule of an n-point line with e
+// elements per lane finds a radix dividing both the remaining length and e,
+// and the n / e lanes of a line divide a wavefront.
+constexpr bool schedule_ok(int n, int e) {
+  if (e <= 0 || n % e != 0 || kWave % (n / e) != 0) return false;
+  int m = n;
+  while (m > 1) {
+    const int r = wave_radix(m, e);
+    if (r < 0) return false;
+    m /= r;
+  }
+  return true;
+}
+
+// Lengths served by the split plane stage only (no class-split or whole-plane
+// kernels): elements per lane of their full-length lines and of their z
+// lines, chosen so that every radix divides the lane's elements (72, 144:
+// 2 2 2 (2) 3 3 on 18; 80, 160: 2 2 2 2 (2) 5 on 10, or 4 4 (2) 5 on 20;
+// 48: 4 4 3 on 12; 256: 8 8 4 on 8 (z: 4 4 4 4 on 4)).
+constexpr bool split_only_length(int n) {
+  return n == 48 || n == 72 || n == 80 || n == 144 || n == 160 || n == 256;
+}
+template <typename T>
+constexpr int split_only_elems(int n) {
+  return n == 48    ? 12
+         : n == 72  ? 18
+         : n == 80  ? (sizeof(T) == 8 ? 10 : 20)
+         : n == 144 ? 18
+         : n == 160 ? (sizeof(T) == 8 ? 10 : 20)
+                    : 8;  // 256
+}
+template <typename T>
+constexpr int split_only_z_elems(int n) {
+  return n == 48    ? 12
+         : n == 72  ? 18
+         : n == 80  ? 10
+         : n == 144 ? 18
+         : n == 160 ? 10
+                    : (sizeof(T) == 8 ? 4 : 8);  // 256
+}
+
 // Lines of the full-tile kernels take the whole lane's elements in their
 // first pass (radix 16 or 24 in registers) so that a line needs one exchange.
 constexpr int full_radix(int m, int e) {
@@ -248,7 +289,11 @@ struct ZShape {
   // FP64 and lengths with a factor 3: narrow lines (more waves in flight);
   // FP32 powers of two: full lines, so a workgroup owns 16 sticks and its
   // [z][stick] row segments are whole 128-byte lines
-  using L = Line<NZ, (sizeof(T) == 4 && NZ % 3 != 0) ? elems_per_lane(NZ) : z_elems_per_lane(NZ)>;
+  using L = Line<NZ, split_only_length(NZ)
+                            ? split_only_z_elems<T>(NZ)
+                            : ((sizeof(T) == 4 && NZ % 3 != 0) ? elems_per_lane(NZ)
+                                                               : z_elems_per_lane(NZ))>;
+  static_assert(schedule_ok(NZ, L::E), "z line schedule");
   static constexpr int NT = 256;
   static constexpr int S = (NT / kWave) * L::PER_WAVE;  // sticks per workgroup
   static constexpr int LZ = padded_len<T>(NZ);
@@ -494,7 +539,9 @@ __device__ __forceinline__ void z_forward_body(const FastArgs<T>& a, int bid) {
 // 96), since 24 FP64 elements need about 230 registers per lane and halve the
 // waves a SIMD holds.
 template <typename T, int N>
-using FullLine = Line<N, (sizeof(T) == 8 && N % 3 == 0) ? 12 : elems_per_lane(N)>;
+using FullLine = Line<N, split_only_length(N)
+                             ? split_only_elems<T>(N)
+                             : ((sizeof(T) == 8 && N % 3 == 0) ? 12 : elems_per_lane(N))>;
 
 template <typename T, int N>
 struct XYShape {
@@ -1005,11 +1052,23 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
 // Lines are FullLine<T, N> (P lanes x E elements); LINES = lines per
 // workgroup. No plane is read more than once.
 
+// Waves of a split-stage workgroup: up to 4, never more lines than the plane
+// has rows (columns), and the exchange regions within 64 KiB.
+template <typename T, int N, typename L>
+constexpr int split_waves() {
+  int nw = N / L::PER_WAVE < 4 ? N / L::PER_WAVE : 4;
+  while (nw > 1 && static_cast<size_t>(nw) * L::PER_WAVE * padded_len<T>(N) * sizeof(Cx<T>) >
+                       64 * 1024) {
+    nw /= 2;
+  }
+  return nw;
+}
+
 template <typename T, int N>
 struct SplitShape {
   using L = FullLine<T, N>;
-  // up to 4 waves, never more lines than the plane has rows (columns)
-  static constexpr int NW = N / L::PER_WAVE < 4 ? N / L::PER_WAVE : 4;
+  static_assert(schedule_ok(N, L::E), "full line schedule");
+  static constexpr int NW = split_waves<T, N, L>();
   static constexpr int NT = NW * kWave;
   static constexpr int LINES = NW * L::PER_WAVE;
   static_assert(NW >= 1 && LINES <= N, "split line shape");
@@ -1757,15 +1816,34 @@ bool fits(int ncol, bool rc) {
          XYShape<T, N>::lds_forward() <= kLdsMax;
 }
 
-bool fast_length(int n) { return n == 32 || n == 64 || n == 96 || n == 128 || n == 192; }
+// Lengths with class-split (and, for some, whole-plane) kernels, and all
+// lengths with z and split-stage kernels.
+bool square_length(int n) { return n == 32 || n == 64 || n == 96 || n == 128 || n == 192; }
+bool fast_length(int n) { return square_length(n) || split_only_length(n); }
+
+#define SPFFT_SQUARE_SWITCH(N, CALL) \
+  switch (N) {                       \
+    case 32: CALL(32); break;        \
+    case 64: CALL(64); break;        \
+    case 96: CALL(96); break;        \
+    case 128: CALL(128); break;      \
+    case 192: CALL(192); break;      \
+    default: throw GPUFFTError();    \
+  }
 
 #define SPFFT_FAST_SWITCH(N, CALL) \
   switch (N) {                     \
     case 32: CALL(32); break;      \
+    case 48: CALL(48); break;      \
     case 64: CALL(64); break;      \
+    case 72: CALL(72); break;      \
+    case 80: CALL(80); break;      \
     case 96: CALL(96); break;      \
     case 128: CALL(128); break;    \
+    case 144: CALL(144); break;    \
+    case 160: CALL(160); break;    \
     case 192: CALL(192); break;    \
+    case 256: CALL(256); break;    \
     default: throw GPUFFTError();  \
   }
 
@@ -1831,12 +1909,17 @@ void fast_z_geometry(int nz, int* sticksPerBlock, int* lineLength, int* padShift
 
 template <typename T>
 bool fast_path_supported(int nx, int ny, int nz, int ncol, bool r2c) {
-  if (!fast_length(nx) || !fast_length(ny) || !fast_length(nz) || ncol < 1) return false;
-  // x != y: the split plane stage (y and x kernels of their own lengths)
-  if (nx != ny) return true;
+  (void)r2c;
+  // the z kernels and the split plane stage serve every fast length
+  return fast_length(nx) && fast_length(ny) && fast_length(nz) && ncol >= 1;
+}
+
+template <typename T>
+bool class_split_supported(int nx, int ny, int ncol, bool r2c) {
+  if (nx != ny || !square_length(nx) || ncol < 1) return false;
   bool ok = false;
 #define SPFFT_FITS(N) ok = fits<T, N>(ncol, r2c)
-  SPFFT_FAST_SWITCH(nx, SPFFT_FITS)
+  SPFFT_SQUARE_SWITCH(nx, SPFFT_FITS)
 #undef SPFFT_FITS
   return ok;
 }
@@ -1859,7 +1942,7 @@ template <typename T>
 void launch_fast_xy_backward(int nx, int ny, const FastArgs<T>& a, hipStream_t stream) {
   if (nx != ny) throw GPUFFTError();
 #define SPFFT_CALL(N) xy_backward_n<T, N>(a, stream)
-  SPFFT_FAST_SWITCH(nx, SPFFT_CALL)
+  SPFFT_SQUARE_SWITCH(nx, SPFFT_CALL)
 #undef SPFFT_CALL
 }
 
@@ -1867,7 +1950,7 @@ template <typename T>
 void launch_fast_xy_forward(int nx, int ny, const FastArgs<T>& a, hipStream_t stream) {
   if (nx != ny) throw GPUFFTError();
 #define SPFFT_CALL(N) xy_forward_n<T, N>(a, stream)
-  SPFFT_FAST_SWITCH(nx, SPFFT_CALL)
+  SPFFT_SQUARE_SWITCH(nx, SPFFT_CALL)
 #undef SPFFT_CALL
 }
 
@@ -1875,7 +1958,7 @@ template <typename T>
 int fast_xy_pitch_a(int n, int ncol) {
   int p = 0;
 #define SPFFT_PITCH(N) p = XYShape<T, N>::pitch_a(ncol)
-  SPFFT_FAST_SWITCH(n, SPFFT_PITCH)
+  SPFFT_SQUARE_SWITCH(n, SPFFT_PITCH)
 #undef SPFFT_PITCH
   return p;
 }
@@ -1893,11 +1976,11 @@ void launch_fast_xy_batch(bool forward, bool r2c, int n, const FastArgs<T>* batc
                           int nz, int maxNcol, hipStream_t stream) {
   if (forward) {
 #define SPFFT_CALL(N) xy_forward_batch_n<T, N>(r2c, batch, ntrans, nz, stream)
-    SPFFT_FAST_SWITCH(n, SPFFT_CALL)
+    SPFFT_SQUARE_SWITCH(n, SPFFT_CALL)
 #undef SPFFT_CALL
   } else {
 #define SPFFT_CALL(N) xy_backward_batch_n<T, N>(r2c, batch, ntrans, nz, maxNcol, stream)
-    SPFFT_FAST_SWITCH(n, SPFFT_CALL)
+    SPFFT_SQUARE_SWITCH(n, SPFFT_CALL)
 #undef SPFFT_CALL
   }
 }
@@ -2133,6 +2216,7 @@ void launch_split_xy_batch(bool forward, bool r2c, int nx, int ny, const FastArg
   template void launch_fast_xy_batch<T>(bool, bool, int, const FastArgs<T>*, int, int,   \
                                         int, hipStream_t);                               \
   template bool fast_path_supported<T>(int, int, int, int, bool);                        \
+  template bool class_split_supported<T>(int, int, int, bool);                           \
   template void fast_z_geometry<T>(int, int*, int*, int*);                               \
   template void launch_fast_z_backward<T>(int, const FastArgs<T>&, hipStream_t);         \
   template void launch_fast_z_forward<T>(int, const FastArgs<T>&, hipStream_t);          \

@@ -86,10 +86,14 @@ inline int fast_pitch(int nst) {
   return (nst + 1 + q - 1) / q * q;
 }
 
-// True when (nx, ny, nz) have fast kernels and their LDS fits for ncol columns
-// (nx != ny: the split plane stage only).
+// True when (nx, ny, nz) have fast kernels: z kernels and the split plane
+// stage for every length in {32, 48, 64, 72, 80, 96, 128, 144, 160, 192, 256}.
 template <typename T>
 bool fast_path_supported(int nx, int ny, int nz, int ncol, bool r2c);
+// True when the class-split plane kernels serve the plane too: nx = ny in
+// {32, 64, 96, 128, 192} and their LDS fits ncol columns.
+template <typename T>
+bool class_split_supported(int nx, int ny, int ncol, bool r2c);
 
 // Row pitch of the backward xy kernel's column tile for ncol columns.
 template <typename T>
