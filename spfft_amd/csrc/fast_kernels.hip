@@ -464,10 +464,22 @@ __device__ __forceinline__ void z_forward_body(const FastArgs<T>& a, int bid) {
   Tw<T>* twl = reinterpret_cast<Tw<T>*>(tile + tw_offset<T>(Z::TW));
   const int s0 = bid * S;
   const int ns = min(S, a.nst - s0);
-  for (int t = threadIdx.x; t < S * NZ; t += NT) {
-    const int s = t % S, zz = t / S;
-    tile[s * LZ + phys<T>(zz)] = s < ns ? a.zbuf[static_cast<size_t>(zz) * a.pitch + s0 + s]
-                                        : Cx<T>{T(0), T(0)};
+  {
+    // thread t holds stick t % S at z = t / S + P k, k < E (S NZ = E NT): all
+    // loads in flight before the tile stores
+    static_assert(S * NZ == L::E * NT && NT % S == 0, "tile load shape");
+    constexpr int ZS = NT / S;  // z rows per pass of the workgroup
+    const int s = threadIdx.x % S, zz0 = threadIdx.x / S;
+    const int sc = s < ns ? s : 0;
+    Cx<T> w[L::E];
+#pragma unroll
+    for (int k = 0; k < L::E; ++k) {
+      w[k] = a.zbuf[static_cast<size_t>(zz0 + ZS * k) * a.pitch + s0 + sc];
+    }
+#pragma unroll
+    for (int k = 0; k < L::E; ++k) {
+      tile[s * LZ + phys<T>(zz0 + ZS * k)] = s < ns ? w[k] : Cx<T>{T(0), T(0)};
+    }
   }
   if (threadIdx.x < NZ) twl[threadIdx.x] = a.twZ[threadIdx.x];
   if (a.hasRuns) {
@@ -1265,12 +1277,21 @@ __device__ __forceinline__ void ysplit_forward_body(const FastArgs<T>& a, int bi
   const int u = u0 + line;
   const int nc = min(S::LINES, a.ncol - u0);
   // LINES columns of every row through the tile (row segments); columns past
-  // ncol are zero
+  // ncol are zero. Thread t holds column t % LINES of rows t / LINES + P k,
+  // k < E (N LINES = E NT), all loads in flight before the tile stores.
   const Cx<T>* yrows = a.ybuf + static_cast<size_t>(z) * N * a.pitchY + u0;
-  for (int t = threadIdx.x; t < N * S::LINES; t += S::NT) {
-    const int y = t / S::LINES, c = t - y * S::LINES;
-    region[y * S::TP + c] =
-        c < nc ? yrows[static_cast<size_t>(y) * a.pitchY + c] : Cx<T>{T(0), T(0)};
+  {
+    static_assert(N * S::LINES == E * S::NT && S::NT % S::LINES == 0, "tile load shape");
+    constexpr int RS = S::NT / S::LINES;  // rows per pass of the workgroup
+    const int c = threadIdx.x % S::LINES, y0 = threadIdx.x / S::LINES;
+    const int cc = c < nc ? c : 0;
+    Cx<T> w[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) w[k] = yrows[static_cast<size_t>(y0 + RS * k) * a.pitchY + cc];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      region[(y0 + RS * k) * S::TP + c] = c < nc ? w[k] : Cx<T>{T(0), T(0)};
+    }
   }
   for (int t = threadIdx.x; t < N; t += S::NT) twl[t] = a.twY[t];
   __syncthreads();
