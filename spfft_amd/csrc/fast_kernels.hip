@@ -732,33 +732,50 @@ __device__ __forceinline__ void xy_backward_body(const FastArgs<T>& a, int bid) 
   __syncthreads();
 
   // x: row m (y = r + R m) holds column u's value at x = colX[u]; absent x
-  // read the zero column. Rows past M (a partial last wave) are computed
-  // from row M - 1 and not stored.
-  const int m = wave * LX::PER_WAVE + lane / LX::P;
-  const int mc = m < M ? m : M - 1;
+  // read the zero column. C2C: line k is row k. R2C: line k carries the
+  // hermitian rows a = 2k and b = 2k + 1 as a + i b, and its result's real
+  // and imaginary parts are the two real rows. Lines past the last (a partial
+  // last wave) are computed from the last one and not stored.
+  static_assert(!RC || M % 2 == 0, "row pairs");
+  constexpr int MX = RC ? M / 2 : M;
+  const int k = wave * LX::PER_WAVE + lane / LX::P;
+  const int kc = k < MX ? k : MX - 1;
+  const int ma = RC ? 2 * kc : kc;
   const int q = lane % LX::P;
   Cx<T> v[LX::E];
 #pragma unroll
   for (int e = 0; e < LX::E; ++e) {
-    Cx<T> x = A[mc * LU + uu[e]];
+    Cx<T> x = A[ma * LU + uu[e]];
     if (RC) {
       // C2R: the imaginary parts of the DC and Nyquist inputs do not
       // contribute to a real signal; the upper half is conjugated
+      Cx<T> xb = A[(ma + 1) * LU + uu[e]];
       const int xx = q + LX::P * e;
-      if (xx == 0 || 2 * xx == N) x.im = T(0);
-      if (2 * xx > N) x = cconj(x);
+      if (xx == 0 || 2 * xx == N) {
+        x.im = T(0);
+        xb.im = T(0);
+      }
+      if (2 * xx > N) {
+        x = cconj(x);
+        xb = cconj(xb);
+      }
+      x = Cx<T>{x.re - xb.im, x.im + xb.re};
     }
     v[e] = x;
   }
   __syncthreads();  // A is dead from here on; its space holds the row lines
-  if (wave * LX::PER_WAVE < M) {
-    Cx<T>* line = A + m * LB;
+  if (wave * LX::PER_WAVE < MX) {
+    Cx<T>* line = A + k * LB;
     wave_fft<T, N, true, 1, LX::E>(v, line, 1, q, twl);
-    if (m < M) {
-      const size_t row = (static_cast<size_t>(z) * N + r + R * m) * N;
+    if (k < MX) {
+      const size_t row = (static_cast<size_t>(z) * N + r + R * ma) * N;
       if (RC) {
+        // row b = row a + R rows
 #pragma unroll
-        for (int e = 0; e < LX::E; ++e) a.out[row + q + LX::P * e] = v[e].re;
+        for (int e = 0; e < LX::E; ++e) {
+          a.out[row + q + LX::P * e] = v[e].re;
+          a.out[row + static_cast<size_t>(R) * N + q + LX::P * e] = v[e].im;
+        }
       } else {
         Cx<T>* out = reinterpret_cast<Cx<T>*>(a.out) + row;
 #pragma unroll
@@ -1088,8 +1105,13 @@ struct SplitShape {
   static constexpr int TP = LINES | 1;         // odd pitch of the [y][LINES] tile
   // the exchange regions, reused as the tile between phases
   static constexpr int REGION = LINES * LB > N * TP ? LINES * LB : N * TP;
-  // workgroups per plane of the row kernels (rows of length N, ny rows)
-  __host__ __device__ static int row_groups(int ny) { return (ny + LINES - 1) / LINES; }
+  // rows per workgroup of the row kernels (rows of length N): R2C packs two
+  // real rows into each complex line
+  __host__ __device__ static constexpr int row_block(bool rc) { return rc ? 2 * LINES : LINES; }
+  // workgroups per plane of the row kernels (ny rows)
+  __host__ __device__ static int row_groups(int ny, bool rc) {
+    return (ny + row_block(rc) - 1) / row_block(rc);
+  }
   // region, twiddles, the filled x = 0 column (R2C backward)
   static constexpr size_t lds() {
     return static_cast<size_t>(tw_offset<T>(REGION)) * sizeof(Cx<T>) + N * sizeof(Tw<T>) +
@@ -1174,31 +1196,44 @@ __device__ __forceinline__ void xsplit_backward_body(const FastArgs<T>& a, int b
   using L = typename S::L;
   constexpr int E = L::E, P = L::P;
   const int ny = a.ny;  // rows of the plane (N is the row length)
-  const int groups = S::row_groups(ny);
+  const int groups = S::row_groups(ny, RC);
   const int z = bid / groups + a.zBegin;
-  const int y0 = (bid % groups) * S::LINES;
+  const int y0 = (bid % groups) * S::row_block(RC);
   if (z >= a.nz) return;
   Cx<T>* region = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
   Tw<T>* twl = reinterpret_cast<Tw<T>*>(region + tw_offset<T>(S::REGION));
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int q = lane % P;
   const int line = wave * L::PER_WAVE + lane / P;
-  const int y = y0 + line;
-  const int yc = y < ny ? y : ny - 1;
+  // C2C: row y0 + line. R2C: rows y0 + 2 line (real part of the line's
+  // result) and y0 + 2 line + 1 (imaginary part); ny is even.
+  const int y = y0 + (RC ? 2 * line : line);
+  const int yc = y < ny ? y : ny - 1 - (RC ? 1 : 0);
   const int ncol = a.ncol;
   const Cx<T>* yrow = a.ybuf + (static_cast<size_t>(z) * ny + yc) * a.pitchY;
   // compact row at its x positions; absent x read column 0 and are zeroed;
-  // R2C: the upper half is the conjugate of x' = N - x
+  // R2C: the upper half is the conjugate of x' = N - x, and the hermitian rows
+  // a and b enter as a + i b (two real outputs from one complex transform)
   Cx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int x = q + P * e;
     const int u = a.xCol[(RC && 2 * x > N) ? N - x : x];
-    const Cx<T> c = yrow[u < ncol ? u : 0];
-    Cx<T> w = u < ncol ? c : Cx<T>{T(0), T(0)};
+    const int uc = u < ncol ? u : 0;
+    Cx<T> w = yrow[uc];
+    if (u >= ncol) w = Cx<T>{T(0), T(0)};
     if (RC) {
-      if (x == 0 || 2 * x == N) w.im = T(0);
-      if (2 * x > N) w = cconj(w);
+      Cx<T> wb = yrow[a.pitchY + uc];
+      if (u >= ncol) wb = Cx<T>{T(0), T(0)};
+      if (x == 0 || 2 * x == N) {
+        w.im = T(0);
+        wb.im = T(0);
+      }
+      if (2 * x > N) {
+        w = cconj(w);
+        wb = cconj(wb);
+      }
+      w = Cx<T>{w.re - wb.im, w.im + wb.re};
     }
     v[e] = w;
   }
@@ -1209,7 +1244,10 @@ __device__ __forceinline__ void xsplit_backward_body(const FastArgs<T>& a, int b
     const size_t row = (static_cast<size_t>(z) * ny + y) * N;
     if (RC) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) a.out[row + q + P * e] = v[e].re;
+      for (int e = 0; e < E; ++e) {
+        a.out[row + q + P * e] = v[e].re;
+        a.out[row + N + q + P * e] = v[e].im;
+      }
     } else {
       Cx<T>* out = reinterpret_cast<Cx<T>*>(a.out) + row;
 #pragma unroll
@@ -1224,22 +1262,24 @@ __device__ __forceinline__ void xsplit_forward_body(const FastArgs<T>& a, int bi
   using L = typename S::L;
   constexpr int E = L::E, P = L::P;
   const int ny = a.ny;  // rows of the plane (N is the row length)
-  const int groups = S::row_groups(ny);
+  const int groups = S::row_groups(ny, RC);
   const int z = bid / groups + a.zBegin;
-  const int y0 = (bid % groups) * S::LINES;
+  const int y0 = (bid % groups) * S::row_block(RC);
   if (z >= a.nz) return;
   Cx<T>* region = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
   Tw<T>* twl = reinterpret_cast<Tw<T>*>(region + tw_offset<T>(S::REGION));
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int q = lane % P;
   const int line = wave * L::PER_WAVE + lane / P;
-  const int y = y0 + line;
-  const int yc = y < ny ? y : ny - 1;
+  // C2C: row y0 + line. R2C: the real rows a = y0 + 2 line and b = a + 1
+  // enter as a + i b; ny is even.
+  const int y = y0 + (RC ? 2 * line : line);
+  const int yc = y < ny ? y : ny - 1 - (RC ? 1 : 0);
   const size_t row = (static_cast<size_t>(z) * ny + yc) * N;
   Cx<T> v[E];
   if (RC) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = Cx<T>{a.in[row + q + P * e], T(0)};
+    for (int e = 0; e < E; ++e) v[e] = Cx<T>{a.in[row + q + P * e], a.in[row + N + q + P * e]};
   } else {
     const Cx<T>* in = reinterpret_cast<const Cx<T>*>(a.in) + row;
 #pragma unroll
@@ -1250,8 +1290,31 @@ __device__ __forceinline__ void xsplit_forward_body(const FastArgs<T>& a, int bi
   for (int e = 0; e < E; ++e) uu[e] = a.xCol[q + P * e];
   for (int t = threadIdx.x; t < N; t += S::NT) twl[t] = a.twX[t];
   __syncthreads();
-  wave_fft<T, N, false, 1, E>(v, region + line * S::LB, 1, q, twl);
-  if (y < ny) {
+  Cx<T>* lr = region + line * S::LB;
+  wave_fft<T, N, false, 1, E>(v, lr, 1, q, twl);
+  if (RC) {
+    // Z = A + i B with A, B the spectra of the real rows a and b:
+    // A[x] = (Z[x] + conj Z[N - x]) / 2, B[x] = -i (Z[x] - conj Z[N - x]) / 2.
+    // The partner of x is held by lane (P - q) % P; it comes through the
+    // line's exchange region (wave-local).
+    wave_sync();
+#pragma unroll
+    for (int e = 0; e < E; ++e) lr[phys<T>(q + P * e)] = v[e];
+    wave_sync();
+    if (y < ny) {
+      Cx<T>* ya = a.ybuf + (static_cast<size_t>(z) * ny + y) * a.pitchY;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int x = q + P * e;
+        if (uu[e] < a.ncol) {
+          const Cx<T> zp = lr[phys<T>(x == 0 ? 0 : N - x)];
+          const T h = T(0.5);
+          ya[uu[e]] = Cx<T>{h * (v[e].re + zp.re), h * (v[e].im - zp.im)};
+          ya[a.pitchY + uu[e]] = Cx<T>{h * (v[e].im + zp.im), h * (zp.re - v[e].re)};
+        }
+      }
+    }
+  } else if (y < ny) {
     Cx<T>* yrow = a.ybuf + (static_cast<size_t>(z) * ny + y) * a.pitchY;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -2144,7 +2207,7 @@ void ysplit_n(bool forward, const FastArgs<T>& a, hipStream_t stream) {
 template <typename T, int NX>
 void xsplit_n(bool forward, const FastArgs<T>& a, hipStream_t stream) {
   using S = SplitShape<T, NX>;
-  const int blocks = S::row_groups(a.ny) * (a.nz - a.zBegin);
+  const int blocks = S::row_groups(a.ny, a.r2c != 0) * (a.nz - a.zBegin);
   if (forward) {
     split_launch<T, NX>(a.r2c ? k_xsplit_forward<T, NX, true> : k_xsplit_forward<T, NX, false>,
                         "k_xsplit_forward", blocks, stream, a);
@@ -2171,7 +2234,7 @@ template <typename T, int NX>
 void xsplit_batch_n(bool forward, bool rc, const FastArgs<T>* batch, int ntrans, int ny, int nz,
                     hipStream_t stream) {
   using S = SplitShape<T, NX>;
-  const int per = S::row_groups(ny) * nz;
+  const int per = S::row_groups(ny, rc) * nz;
   if (forward) {
     split_launch<T, NX>(rc ? k_xsplit_forward_batch<T, NX, true>
                            : k_xsplit_forward_batch<T, NX, false>,
