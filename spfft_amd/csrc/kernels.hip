@@ -136,6 +136,28 @@ __device__ __forceinline__ void zero_lds(Cx<T>* a, int count) {
   for (int t = threadIdx.x; t < count; t += blockDim.x) a[t] = Cx<T>{T(0), T(0)};
 }
 
+// Thread-strided loop over [begin, end) with the global loads of kUnroll
+// iterations in flight together: load(t) issues iteration t's loads and
+// returns their results, use(t, x) consumes them (LDS or global stores).
+constexpr int kUnroll = 8;
+template <typename Load, typename Use>
+__device__ __forceinline__ void batched_loop(int begin, int end, Load load, Use use) {
+  for (int t0 = begin + static_cast<int>(threadIdx.x); t0 < end;
+       t0 += kUnroll * static_cast<int>(blockDim.x)) {
+    decltype(load(t0)) x[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int t = t0 + u * static_cast<int>(blockDim.x);
+      if (t < end) x[u] = load(t);
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int t = t0 + u * static_cast<int>(blockDim.x);
+      if (t < end) use(t, x[u]);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- backward
 
 template <typename T>
@@ -149,11 +171,32 @@ __global__ void __launch_bounds__(kBlock) k_z_backward(StageArgs<T> a) {
   __syncthreads();
   const int e0 = a.stickBegin[s0], e1 = a.stickBegin[s0 + ns];
   const int base = s0 * nz;
-  for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    const int slot = a.entrySlot[e];
-    if (e + 1 < e1 && a.entrySlot[e + 1] == slot) continue;  // repeated triplet: last one wins
-    const size_t v = static_cast<size_t>(a.entryValue[e]);
-    A[slot - base] = Cx<T>{a.in[2 * v], a.in[2 * v + 1]};
+  const Cx<T>* in = reinterpret_cast<const Cx<T>*>(a.in);
+  // entries of kUnroll iterations, then their values, all in flight together;
+  // a repeated triplet's earlier copies are skipped (the last one wins)
+  for (int f0 = e0 + static_cast<int>(threadIdx.x); f0 < e1;
+       f0 += kUnroll * static_cast<int>(blockDim.x)) {
+    int slot[kUnroll], val[kUnroll];
+    bool keep[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int e = f0 + u * static_cast<int>(blockDim.x);
+      keep[u] = false;
+      if (e < e1) {
+        slot[u] = a.entrySlot[e];
+        val[u] = a.entryValue[e];
+        keep[u] = !(e + 1 < e1 && a.entrySlot[e + 1] == slot[u]);
+      }
+    }
+    Cx<T> x[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (keep[u]) x[u] = in[val[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (keep[u]) A[slot[u] - base] = x[u];
+    }
   }
   __syncthreads();
   if (a.r2c && s0 == 0 && a.zeroZeroStick == 0) {
@@ -178,9 +221,14 @@ __global__ void __launch_bounds__(kBlock) k_y_backward(StageArgs<T> a) {
   zero_lds(A, nc * ny);
   __syncthreads();
   const Cx<T>* row = a.zbuf + static_cast<size_t>(z) * a.nst;
-  for (int s = a.colBegin[u0] + threadIdx.x; s < a.colBegin[u0 + nc]; s += blockDim.x) {
-    A[(a.stickCol[s] - u0) * ny + a.stickY[s]] = row[s];
-  }
+  struct Gathered {
+    int at;
+    Cx<T> x;
+  };
+  batched_loop(
+      a.colBegin[u0], a.colBegin[u0 + nc],
+      [&](int s) { return Gathered{(a.stickCol[s] - u0) * ny + a.stickY[s], row[s]}; },
+      [&](int, const Gathered& g) { A[g.at] = g.x; });
   __syncthreads();
   if (a.r2c && u0 == 0 && a.colX[0] == 0) {
     hermitian_fill_line(A, ny);
@@ -204,10 +252,17 @@ __global__ void __launch_bounds__(kBlock) k_x_backward(StageArgs<T> a) {
   zero_lds(A, nr * nx);
   __syncthreads();
   const Cx<T>* src = a.ybuf + (static_cast<size_t>(z) * ny + y0) * ncol;
-  for (int t = threadIdx.x; t < nr * ncol; t += blockDim.x) {
-    const int yl = t / ncol, u = t - yl * ncol;
-    A[yl * nx + a.colX[u]] = src[t];
-  }
+  struct Gathered {
+    int at;
+    Cx<T> x;
+  };
+  batched_loop(
+      0, nr * ncol,
+      [&](int t) {
+        const int yl = t / ncol, u = t - yl * ncol;
+        return Gathered{yl * nx + a.colX[u], src[t]};
+      },
+      [&](int, const Gathered& g) { A[g.at] = g.x; });
   __syncthreads();
   if (a.r2c) {
     // Complete the hermitian half along x; the imaginary parts of the DC and
@@ -245,18 +300,24 @@ __global__ void __launch_bounds__(kBlock) k_x_forward(StageArgs<T> a) {
   const int nr = min(a.batch, ny - y0);
   const size_t inBase = (static_cast<size_t>(z) * ny + y0) * nx;
   if (a.r2c) {
-    for (int t = threadIdx.x; t < nr * nx; t += blockDim.x) A[t] = Cx<T>{a.in[inBase + t], T(0)};
+    batched_loop(
+        0, nr * nx, [&](int t) { return a.in[inBase + t]; },
+        [&](int t, T x) { A[t] = Cx<T>{x, T(0)}; });
   } else {
     const Cx<T>* in = reinterpret_cast<const Cx<T>*>(a.in);
-    for (int t = threadIdx.x; t < nr * nx; t += blockDim.x) A[t] = in[inBase + t];
+    batched_loop(
+        0, nr * nx, [&](int t) { return in[inBase + t]; }, [&](int t, Cx<T> x) { A[t] = x; });
   }
   __syncthreads();
   const Cx<T>* r = lds_fft<T, false>(A, B, a.fft, nr, a.tw);
   Cx<T>* dst = a.ybuf + (static_cast<size_t>(z) * ny + y0) * ncol;
-  for (int t = threadIdx.x; t < nr * ncol; t += blockDim.x) {
-    const int yl = t / ncol, u = t - yl * ncol;
-    dst[t] = r[yl * nx + a.colX[u]];
-  }
+  batched_loop(
+      0, nr * ncol,
+      [&](int t) {
+        const int yl = t / ncol, u = t - yl * ncol;
+        return yl * nx + a.colX[u];
+      },
+      [&](int t, int at) { dst[t] = r[at]; });
 }
 
 template <typename T>
@@ -267,16 +328,20 @@ __global__ void __launch_bounds__(kBlock) k_y_forward(StageArgs<T> a) {
   const int z = blockIdx.y;
   const int u0 = blockIdx.x * a.batch;
   const int nc = min(a.batch, a.ncol - u0);
-  for (int t = threadIdx.x; t < nc * ny; t += blockDim.x) {
-    const int ul = t % nc, y = t / nc;
-    A[ul * ny + y] = a.ybuf[(static_cast<size_t>(z) * ny + y) * a.ncol + u0 + ul];
-  }
+  batched_loop(
+      0, nc * ny,
+      [&](int t) {
+        const int ul = t % nc, y = t / nc;
+        return a.ybuf[(static_cast<size_t>(z) * ny + y) * a.ncol + u0 + ul];
+      },
+      [&](int t, Cx<T> x) { A[(t % nc) * ny + t / nc] = x; });
   __syncthreads();
   const Cx<T>* r = lds_fft<T, false>(A, B, a.fft, nc, a.tw);
   Cx<T>* row = a.zbuf + static_cast<size_t>(z) * a.nst;
-  for (int s = a.colBegin[u0] + threadIdx.x; s < a.colBegin[u0 + nc]; s += blockDim.x) {
-    row[s] = r[(a.stickCol[s] - u0) * ny + a.stickY[s]];
-  }
+  batched_loop(
+      a.colBegin[u0], a.colBegin[u0 + nc],
+      [&](int s) { return (a.stickCol[s] - u0) * ny + a.stickY[s]; },
+      [&](int s, int at) { row[s] = r[at]; });
 }
 
 template <typename T>
@@ -286,20 +351,22 @@ __global__ void __launch_bounds__(kBlock) k_z_forward(StageArgs<T> a) {
   Cx<T>* B = A + a.batch * nz;
   const int s0 = blockIdx.x * a.batch;
   const int ns = min(a.batch, a.nst - s0);
-  for (int t = threadIdx.x; t < ns * nz; t += blockDim.x) {
-    const int sl = t % ns, z = t / ns;
-    A[sl * nz + z] = a.zbuf[static_cast<size_t>(z) * a.nst + s0 + sl];
-  }
+  batched_loop(
+      0, ns * nz,
+      [&](int t) { return a.zbuf[static_cast<size_t>(t / ns) * a.nst + s0 + t % ns]; },
+      [&](int t, Cx<T> x) { A[(t % ns) * nz + t / ns] = x; });
   __syncthreads();
   const Cx<T>* r = lds_fft<T, false>(A, B, a.fft, ns, a.tw);
   const int base = s0 * nz;
-  for (int e = a.stickBegin[s0] + threadIdx.x; e < a.stickBegin[s0 + ns]; e += blockDim.x) {
-    Cx<T> c = r[a.entrySlot[e] - base];
-    if (a.useScale) c = cscale(c, a.scale);
-    const size_t v = static_cast<size_t>(a.entryValue[e]);
-    a.out[2 * v] = c.re;
-    a.out[2 * v + 1] = c.im;
-  }
+  Cx<T>* out = reinterpret_cast<Cx<T>*>(a.out);
+  batched_loop(
+      a.stickBegin[s0], a.stickBegin[s0 + ns],
+      [&](int e) { return make_int2(a.entrySlot[e] - base, a.entryValue[e]); },
+      [&](int, int2 sv) {
+        Cx<T> c = r[sv.x];
+        if (a.useScale) c = cscale(c, a.scale);
+        out[sv.y] = c;
+      });
 }
 
 template <typename T>
