@@ -278,10 +278,11 @@ def launch_floor(torch, kernel_us):
 
 
 def plain_lengths_bench(args, sp, torch, np):
-    """Single C2C FP64 transforms at lengths without fixed-length kernels
-    (the generic stage kernels serve them), sphere radius 0.3125 N (40 at
-    128): backward+forward pairs/s and the fraction of the HBM roofline of the
-    pair's compulsory bytes, rotating over 8 transforms."""
+    """Single C2C FP64 transforms at other plane-wave lengths, sphere radius
+    0.3125 N (40 at 128): 120 and 100 (generic stage kernels) and 144 (z
+    kernels and the split plane stage). Backward+forward pairs/s and the
+    fraction of the HBM roofline of the pair's compulsory bytes, rotating over
+    8 transforms; `kernels` names the kernels one pair launched."""
     from tests.workloads import complex_values, sphere_triplets
     out = {}
     for n in (120, 100, 144):
@@ -311,10 +312,18 @@ def plain_lengths_bench(args, sp, torch, np):
             pair(i)
         torch.cuda.synchronize()
         rate = steps / (time.perf_counter() - t0)
+        sp.lib.spfft_amd_profiling_enable(1)
+        pair(0)
+        torch.cuda.synchronize()
+        kernels = sorted(read_profile(sp.lib))
+        sp.lib.spfft_amd_profiling_enable(0)
+        path = "fixed-length" if any(k.startswith("k_z_backward_") and k != "k_z_backward"
+                                     for k in kernels) else "generic"
         pair_bytes = 2 * (20 * len(t) + 16 * n ** 3)
         out[f"{n}^3"] = {"values": len(t), "pairs_per_s": rate,
-                         "frac_hbm_wall": pair_bytes * rate / 1e9 / HBM_PEAK_GBS}
-        log(f"plain C2C {n}^3 r={0.3125 * n:.1f} ({len(t)} values, generic kernels): "
+                         "frac_hbm_wall": pair_bytes * rate / 1e9 / HBM_PEAK_GBS,
+                         "path": path, "kernels": kernels}
+        log(f"plain C2C {n}^3 r={0.3125 * n:.1f} ({len(t)} values, {path} kernels): "
             f"{rate:.1f} pairs/s, {out[f'{n}^3']['frac_hbm_wall'] * 100:.1f}% of HBM roofline")
         del trs, ins, outs
         torch.cuda.empty_cache()
