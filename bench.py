@@ -384,8 +384,11 @@ def traffic_summary(fetch_dir, write_dir, copy_mib, precision, command):
     script (FETCH_SIZE, then WRITE_SIZE, both in KiB per dispatch), each
     scaled by what the counter reads for the device-to-device copy of
     --copy-mib MiB that the profiled runs include (the runtime's plain copy
-    kernel: it reads and writes exactly that many bytes). Writes
-    profiles/traffic_<kernel>_<precision>.json for the library's kernels."""
+    kernel: it reads and writes exactly that many bytes). Merges one record
+    per library kernel into profiles/traffic_kernels.json under the key
+    <kernel>_<length>_<precision> (R2C instances as <kernel>_r2c; precision
+    and length from the kernel's template arguments); keys of earlier runs
+    that this run does not measure are kept."""
     import statistics
     expect_kib = copy_mib * 1024.0
     per = {}
@@ -403,23 +406,35 @@ def traffic_summary(fetch_dir, write_dir, copy_mib, precision, command):
             base = n.split("<")[0]
             if ", true>" in n.split("(")[0]:  # R2C variants are profiled as <name>_r2c
                 base = base.replace("_fast", "") + "_r2c"
-            per.setdefault(base, {}).setdefault(counter, []).append(v)
-    out = {}
+            targs = n.split("(")[0].split("<", 1)[1].split(">")[0].split(",") if "<" in n else []
+            prec = targs[0].strip() if targs else precision
+            length = targs[1].strip() if len(targs) > 1 else "any"
+            key = f"{base}_{length}_{prec}"
+            per.setdefault(key, {}).setdefault(counter, []).append(v)
+    path = os.path.join(ROOT, "profiles", "traffic_kernels.json")
+    doc = {"runs": {}, "kernels": {}}
+    if os.path.exists(path):
+        with open(path) as fh:
+            doc = json.load(fh)
+    run = f"{fetch_dir} + {write_dir}"
+    doc["runs"][run] = {"command": command, "copy_mib": copy_mib,
+                        "fetch_scale": scale["FETCH_SIZE"], "write_scale": scale["WRITE_SIZE"]}
+    out = doc["kernels"]
     for k, d in per.items():
         if "FETCH_SIZE" not in d or "WRITE_SIZE" not in d:
             continue
         f = statistics.mean(d["FETCH_SIZE"])
         w = statistics.mean(d["WRITE_SIZE"])
         out[k] = {
-            "fetch_kib_per_dispatch": f, "write_kib_per_dispatch": w,
-            "fetch_scale": scale["FETCH_SIZE"], "write_scale": scale["WRITE_SIZE"],
-            "bytes_per_launch": 1024.0 * (f * scale["FETCH_SIZE"] + w * scale["WRITE_SIZE"]),
-            "dispatches": len(d["FETCH_SIZE"]), "copy_mib": copy_mib, "command": command,
+            "fetch_kib_per_dispatch": round(f, 1), "write_kib_per_dispatch": round(w, 1),
+            "bytes_per_launch": round(1024.0 * (f * scale["FETCH_SIZE"] + w * scale["WRITE_SIZE"])),
+            "dispatches": len(d["FETCH_SIZE"]), "run": run,
         }
-        with open(os.path.join(ROOT, "profiles", f"traffic_{k}_{precision}.json"), "w") as fh:
-            json.dump(out[k], fh, indent=1)
-        log(f"{k:28s} fetch {f:10.1f} KiB x{scale['FETCH_SIZE']:.2f}  write {w:10.1f} KiB "
+        log(f"{k:40s} fetch {f:10.1f} KiB x{scale['FETCH_SIZE']:.2f}  write {w:10.1f} KiB "
             f"x{scale['WRITE_SIZE']:.2f}  -> {out[k]['bytes_per_launch'] / 1e6:8.2f} MB/launch")
+    doc["kernels"] = dict(sorted(out.items()))
+    with open(path, "w") as fh:
+        json.dump(doc, fh, indent=1)
     return out
 
 
@@ -441,7 +456,7 @@ def main():
                          "(the calibration dispatches for --pmc FETCH_SIZE / WRITE_SIZE runs)")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc outputs (FETCH_SIZE, WRITE_SIZE) of "
-                         "runs with --copy-mib into profiles/traffic_*.json and exit")
+                         "runs with --copy-mib into profiles/traffic_kernels.json and exit")
     ap.add_argument("--traffic-command", default="",
                     help="the profiling command recorded in the traffic summaries")
     args = ap.parse_args()
@@ -554,10 +569,11 @@ def main():
     # Measured HBM bytes per launch of the dominant kernel, from rocprofv3
     # FETCH_SIZE / WRITE_SIZE runs, when a summary for this kernel exists.
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dominant}_{args.precision}.json")
+    tfile = os.path.join(ROOT, "profiles", "traffic_kernels.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            traffic = json.load(f)["kernels"].get(f"{dominant}_128_{args.precision}", {}).get(
+                "bytes_per_launch")
 
     result = {
         "metric": "bwd+fwd pairs/s @128^3 FP64 (sparse sphere r=40, C2C)",
