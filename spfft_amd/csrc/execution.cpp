@@ -445,7 +445,12 @@ FastArgs<T> ExecutionGPU<T>::fast_args() const {
   a.pitchT = pitchT_;
   a.resYM = tab_->resYM.data();
   a.ybuf = reinterpret_cast<Cx<T>*>(grid_->work_gpu());
-  a.pitchY = split_pitch<T>(numCols_);
+  a.pitchY = split_pitch<T>(numCols_, map_->dimX);
+  // the split stage's column buffer must fit the grid's work buffer
+  if (static_cast<size_t>(map_->dimZ) * map_->dimY * a.pitchY >
+      static_cast<size_t>(grid_->max_dim_x()) * grid_->max_dim_y() * grid_->max_dim_z()) {
+    throw InternalError();
+  }
   a.twX = tab_->twX.data();
   a.twY = tab_->twY.data();
   a.twZ = tab_->twZ.data();

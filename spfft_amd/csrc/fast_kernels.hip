@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <type_traits>
 #include <unordered_map>
 
 #include "fast_kernels.hpp"
@@ -60,7 +61,7 @@ constexpr int wave_radix(int m, int e) {
 // True when every pass of the wave-level schedule of an n-point line with e
 // elements per lane finds a radix dividing both the remaining length and e,
 // and the n / e lanes of a line divide a wavefront.
-constexpr bool schedule_ok(int n, int e) {
+constexpr bool dividing_schedule(int n, int e) {
   if (e <= 0 || n % e != 0 || kWave % (n / e) != 0) return false;
   int m = n;
   while (m > 1) {
@@ -71,30 +72,119 @@ constexpr bool schedule_ok(int n, int e) {
   return true;
 }
 
+// Mixed schedule, for lines whose lane element count cannot take every radix
+// (e = 15 with a factor 8 in n, e = 25 or 27 with a factor 4): the first pass
+// reads the lane's own elements and the last pass leaves them in natural
+// order, so their radices divide e; the passes between them read their
+// butterflies' inputs from the line's LDS region anyway, so a lane there
+// takes ceil((n / r) / P) butterflies of any radix r (the surplus ones of the
+// last lanes are masked). Middle radices are picked to keep the fewest
+// elements per lane (r * ceil((n / r) / P)), the larger radix on ties.
+struct MixedSchedule {
+  int radix[16] = {};
+  int count = 0;
+  bool ok = false;
+};
+constexpr int dividing_pick(int m, int e) {
+  return (m % 5 == 0 && e % 5 == 0)   ? 5
+         : (m % 3 == 0 && e % 3 == 0) ? 3
+         : (m % 8 == 0 && e % 8 == 0) ? 8
+         : (m % 4 == 0 && e % 4 == 0) ? 4
+         : (m % 2 == 0 && e % 2 == 0) ? 2
+                                      : -1;
+}
+constexpr MixedSchedule mixed_schedule(int n, int e) {
+  MixedSchedule s;
+  if (e <= 0 || n % e != 0 || kWave % (n / e) != 0) return s;
+  const int p = n / e;
+  const int first = dividing_pick(n, e);
+  if (first < 0) return s;
+  const int last = dividing_pick(n / first, e);
+  if (last < 0) return s;
+  s.radix[s.count++] = first;
+  int m = n / (first * last);
+  const int cand[5] = {8, 4, 2, 3, 5};
+  while (m > 1) {
+    int best = -1, bestCost = 1 << 30;
+    for (int i = 0; i < 5; ++i) {
+      const int r = cand[i];
+      if (m % r != 0) continue;
+      const int cost = r * ((n / r + p - 1) / p);
+      if (cost < bestCost || (cost == bestCost && r > best)) {
+        best = r;
+        bestCost = cost;
+      }
+    }
+    if (best < 0 || s.count >= 15) return s;
+    s.radix[s.count++] = best;
+    m /= best;
+  }
+  s.radix[s.count++] = last;
+  s.ok = true;
+  return s;
+}
+// Product of the radices of the passes before `pass`.
+constexpr int mixed_ns(int n, int e, int pass) {
+  const MixedSchedule s = mixed_schedule(n, e);
+  int ns = 1;
+  for (int i = 0; i < pass; ++i) ns *= s.radix[i];
+  return ns;
+}
+// Elements a lane holds in the busiest pass of the mixed schedule.
+constexpr int mixed_elems(int n, int e) {
+  const MixedSchedule s = mixed_schedule(n, e);
+  int most = e;
+  for (int i = 0; i < s.count; ++i) {
+    const int r = s.radix[i];
+    const int per = r * ((n / r + n / e - 1) / (n / e));
+    most = per > most ? per : most;
+  }
+  return most;
+}
+
+// A line of n points with e elements per lane has a wave-level schedule.
+constexpr bool schedule_ok(int n, int e) {
+  return dividing_schedule(n, e) || mixed_schedule(n, e).ok;
+}
+
 // Lengths served by the split plane stage only (no class-split or whole-plane
 // kernels): elements per lane of their full-length lines and of their z
 // lines, chosen so that every radix divides the lane's elements (72, 144:
 // 2 2 2 (2) 3 3 on 18; 80, 160: 2 2 2 2 (2) 5 on 10, or 4 4 (2) 5 on 20;
 // 48: 4 4 3 on 12; 256: 8 8 4 on 8 (z: 4 4 4 4 on 4)).
+// Mixed schedules (see mixed_schedule): 60 = 4 x 15 (5 4 3), 120 = 8 x 15
+// (5 8 3), 100 = 4 x 25 (5 2 2 5), 108 = 4 x 27 (3 3 4 3), 216 = 8 x 27
+// (3 3 4 2 3).
 constexpr bool split_only_length(int n) {
-  return n == 48 || n == 72 || n == 80 || n == 144 || n == 160 || n == 256;
+  return n == 48 || n == 60 || n == 72 || n == 80 || n == 100 || n == 108 || n == 120 ||
+         n == 144 || n == 160 || n == 216 || n == 256;
 }
 template <typename T>
 constexpr int split_only_elems(int n) {
   return n == 48    ? 12
+         : n == 60  ? 15
          : n == 72  ? 18
          : n == 80  ? (sizeof(T) == 8 ? 10 : 20)
+         : n == 100 ? 25
+         : n == 108 ? 27
+         : n == 120 ? 15
          : n == 144 ? 18
          : n == 160 ? (sizeof(T) == 8 ? 10 : 20)
+         : n == 216 ? 27
                     : 8;  // 256
 }
 template <typename T>
 constexpr int split_only_z_elems(int n) {
   return n == 48    ? 12
+         : n == 60  ? 15
          : n == 72  ? 18
          : n == 80  ? 10
+         : n == 100 ? 25
+         : n == 108 ? 27
+         : n == 120 ? 15
          : n == 144 ? 18
          : n == 160 ? 10
+         : n == 216 ? 27
                     : (sizeof(T) == 8 ? 4 : 8);  // 256
 }
 
@@ -203,10 +293,72 @@ struct NoHook {
 // The twiddle table holds W_{N * TWS}. NS = product of the radices already
 // applied. `hook` runs once, after the first pass has written its outputs to
 // LDS (the line's registers are free then): callers issue loads there.
+// Pass PASS of the mixed schedule (see mixed_schedule). v: the lane's E
+// elements (input of the first pass, output of the last); between passes the
+// line lives in its LDS region.
+template <typename T, int N, bool BWD, int TWS, int E, int PASS = 0>
+__device__ __forceinline__ void wave_fft_mixed(Cx<T>* v, Cx<T>* line, int es, int q,
+                                               const Tw<T>* __restrict__ tw) {
+  constexpr MixedSchedule S = mixed_schedule(N, E);
+  static_assert(S.ok, "no mixed schedule");
+  constexpr int PL = N / E;
+  constexpr int R = S.radix[PASS];
+  constexpr int NB = N / R;  // butterflies of the pass
+  constexpr bool FIRST = PASS == 0, LAST = PASS == S.count - 1;
+  static_assert(!(FIRST || LAST) || E % R == 0, "first and last radices divide E");
+  constexpr int BL = FIRST || LAST ? E / R : (NB + PL - 1) / PL;  // butterflies per lane
+  constexpr int NS = mixed_ns(N, E, PASS);
+  constexpr int STEP = N / (NS * R) * TWS;
+  Cx<T> u[BL][R];
+#pragma unroll
+  for (int b = 0; b < BL; ++b) {
+    const int j = q + PL * b;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if constexpr (FIRST) {
+        u[b][r] = v[b + r * (E / R)];
+      } else {
+        const int jj = j < NB ? j : 0;
+        u[b][r] = line[slot<T, 0>(jj + r * NB) * es];
+      }
+    }
+  }
+  if constexpr (!FIRST) wave_sync();
+#pragma unroll
+  for (int b = 0; b < BL; ++b) {
+    const int j = q + PL * b;
+    if constexpr (NS > 1) {
+      const int k = j % NS;
+#pragma unroll
+      for (int r = 1; r < R; ++r) u[b][r] = cmul(u[b][r], twiddle<BWD>(tw, k * r * STEP));
+    }
+    Butterfly<T, R, BWD>::run(u[b], tw, N);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + r * (E / R)] = u[b][r];
+    } else {
+      const int o = (j / NS) * NS * R + (j % NS);
+      if (FIRST || j < NB) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) line[slot<T, 0>(o + r * NS) * es] = u[b][r];
+      }
+    }
+  }
+  if constexpr (!LAST) {
+    wave_sync();
+    wave_fft_mixed<T, N, BWD, TWS, E, PASS + 1>(v, line, es, q, tw);
+  }
+}
+
 template <typename T, int N, bool BWD, int TWS, int E = elems_per_lane(N), int NS = 1,
           int PE = 0, typename Hook = NoHook>
 __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
                                          const Tw<T>* __restrict__ tw, Hook hook = Hook()) {
+  if constexpr (NS == 1 && PE == 0 && !dividing_schedule(N, E)) {
+    static_assert(std::is_same<Hook, NoHook>::value, "mixed schedules take no hook");
+    wave_fft_mixed<T, N, BWD, TWS, E>(v, line, es, q, tw);
+    return;
+  } else {
   constexpr int PL = N / E;
   constexpr int R = PE > 0 ? full_radix(N / NS, E) : wave_radix(N / NS, E);
   static_assert(R > 0, "no radix divides the lane's element count");
@@ -247,6 +399,7 @@ __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
     }
     wave_sync();
     wave_fft<T, N, BWD, TWS, E, NS * R, PE>(v, line, es, q, tw);
+  }
   }
 }
 
@@ -1919,14 +2072,19 @@ bool fast_length(int n) { return square_length(n) || split_only_length(n); }
   switch (N) {                     \
     case 32: CALL(32); break;      \
     case 48: CALL(48); break;      \
+    case 60: CALL(60); break;      \
     case 64: CALL(64); break;      \
     case 72: CALL(72); break;      \
     case 80: CALL(80); break;      \
     case 96: CALL(96); break;      \
+    case 100: CALL(100); break;    \
+    case 108: CALL(108); break;    \
+    case 120: CALL(120); break;    \
     case 128: CALL(128); break;    \
     case 144: CALL(144); break;    \
     case 160: CALL(160); break;    \
     case 192: CALL(192); break;    \
+    case 216: CALL(216); break;    \
     case 256: CALL(256); break;    \
     default: throw GPUFFTError();  \
   }
@@ -2247,9 +2405,13 @@ void xsplit_batch_n(bool forward, bool rc, const FastArgs<T>* batch, int ntrans,
 }
 
 template <typename T>
-int split_pitch(int ncol) {
-  const int q = static_cast<int>(128 / sizeof(Cx<T>));  // 128-byte row segments
-  return (ncol + q - 1) / q * q;
+int split_pitch(int ncol, int nx) {
+  // 128-byte row segments, but never wider than a row of the grid: the
+  // buffer lives in the grid's work buffer of nx * ny * nz elements (ncol <=
+  // nx, since columns are distinct x)
+  const int q = static_cast<int>(128 / sizeof(Cx<T>));
+  const int p = (ncol + q - 1) / q * q;
+  return p <= nx ? p : nx;
 }
 
 template <typename T>
@@ -2290,7 +2452,7 @@ void launch_split_xy_batch(bool forward, bool r2c, int nx, int ny, const FastArg
   template void launch_plane_xy_batch<T>(bool, int, const FastArgs<T>*, int, int, int,   \
                                          int, hipStream_t);                              \
   template int plane_xy_slots<T>(bool, int);                                             \
-  template int split_pitch<T>(int);                                                      \
+  template int split_pitch<T>(int, int);                                                 \
   template void launch_split_xy<T>(bool, int, int, const FastArgs<T>&, hipStream_t);     \
   template void launch_split_xy_batch<T>(bool, bool, int, int, const FastArgs<T>*, int,  \
                                          int, int, hipStream_t);                         \

@@ -133,8 +133,10 @@ int plane_xy_slots(bool forward, int n);
 // intermediate; it serves large lengths better than the class-split kernels,
 // which read each plane once per residue class. Single transforms (planes
 // [zBegin, nz)) and batches.
+// Row pitch of the compact column buffer [z][y][column]: ncol rounded up to
+// 128 bytes, at most nx (the buffer must fit nx * ny * nz elements).
 template <typename T>
-int split_pitch(int ncol);
+int split_pitch(int ncol, int nx);
 template <typename T>
 void launch_split_xy(bool forward, int nx, int ny, const FastArgs<T>& a, hipStream_t stream);
 template <typename T>
