@@ -45,9 +45,14 @@ constexpr int z_elems_per_lane(int n) { return n % 3 == 0 ? 12 : 4; }
 template <int N, int EL = elems_per_lane(N)>
 struct Line {
   static constexpr int E = EL;
-  static constexpr int P = N / E;      // lanes per line
-  static constexpr int PER_WAVE = kWave / P;
-  static_assert(N % E == 0 && kWave % P == 0, "line shape");
+  static constexpr int P = N / E;             // lanes per line
+  static constexpr int PER_WAVE = kWave / P;  // whole lines per wave
+  // Lanes past the last whole line of a wave (P not dividing the wave): they
+  // run a dummy line in an exchange region of their own and store nothing.
+  static constexpr bool SPARE = PER_WAVE * P < kWave;
+  static_assert(N % E == 0 && P <= kWave, "line shape");
+  // True for the spare lanes of a wave.
+  __device__ static bool spare(int lane) { return SPARE && lane / P >= PER_WAVE; }
 };
 
 constexpr int wave_radix(int m, int e) {
@@ -60,9 +65,9 @@ constexpr int wave_radix(int m, int e) {
 }
 // True when every pass of the wave-level schedule of an n-point line with e
 // elements per lane finds a radix dividing both the remaining length and e,
-// and the n / e lanes of a line divide a wavefront.
+// and the n / e lanes of a line fit a wavefront.
 constexpr bool dividing_schedule(int n, int e) {
-  if (e <= 0 || n % e != 0 || kWave % (n / e) != 0) return false;
+  if (e <= 0 || n % e != 0 || n / e > kWave) return false;
   int m = n;
   while (m > 1) {
     const int r = wave_radix(m, e);
@@ -95,7 +100,7 @@ constexpr int dividing_pick(int m, int e) {
 }
 constexpr MixedSchedule mixed_schedule(int n, int e) {
   MixedSchedule s;
-  if (e <= 0 || n % e != 0 || kWave % (n / e) != 0) return s;
+  if (e <= 0 || n % e != 0 || n / e > kWave) return s;
   const int p = n / e;
   const int first = dividing_pick(n, e);
   if (first < 0) return s;
@@ -130,18 +135,6 @@ constexpr int mixed_ns(int n, int e, int pass) {
   for (int i = 0; i < pass; ++i) ns *= s.radix[i];
   return ns;
 }
-// Elements a lane holds in the busiest pass of the mixed schedule.
-constexpr int mixed_elems(int n, int e) {
-  const MixedSchedule s = mixed_schedule(n, e);
-  int most = e;
-  for (int i = 0; i < s.count; ++i) {
-    const int r = s.radix[i];
-    const int per = r * ((n / r + n / e - 1) / (n / e));
-    most = per > most ? per : most;
-  }
-  return most;
-}
-
 // A line of n points with e elements per lane has a wave-level schedule.
 constexpr bool schedule_ok(int n, int e) {
   return dividing_schedule(n, e) || mixed_schedule(n, e).ok;
@@ -154,10 +147,11 @@ constexpr bool schedule_ok(int n, int e) {
 // 48: 4 4 3 on 12; 256: 8 8 4 on 8 (z: 4 4 4 4 on 4)).
 // Mixed schedules (see mixed_schedule): 60 = 4 x 15 (5 4 3), 120 = 8 x 15
 // (5 8 3), 100 = 4 x 25 (5 2 2 5), 108 = 4 x 27 (3 3 4 3), 216 = 8 x 27
-// (3 3 4 2 3).
+// (3 3 4 2 3); with spare lanes (6 and 12 lanes per line): 90 = 6 x 15
+// (5 3 2 3), 180 = 12 x 15 (5 3 4 3).
 constexpr bool split_only_length(int n) {
-  return n == 48 || n == 60 || n == 72 || n == 80 || n == 100 || n == 108 || n == 120 ||
-         n == 144 || n == 160 || n == 216 || n == 256;
+  return n == 48 || n == 60 || n == 72 || n == 80 || n == 90 || n == 100 || n == 108 ||
+         n == 120 || n == 144 || n == 160 || n == 180 || n == 216 || n == 256;
 }
 template <typename T>
 constexpr int split_only_elems(int n) {
@@ -165,11 +159,13 @@ constexpr int split_only_elems(int n) {
          : n == 60  ? 15
          : n == 72  ? 18
          : n == 80  ? (sizeof(T) == 8 ? 10 : 20)
+         : n == 90  ? 15
          : n == 100 ? 25
          : n == 108 ? 27
          : n == 120 ? 15
          : n == 144 ? 18
          : n == 160 ? (sizeof(T) == 8 ? 10 : 20)
+         : n == 180 ? 15
          : n == 216 ? 27
                     : 8;  // 256
 }
@@ -179,11 +175,13 @@ constexpr int split_only_z_elems(int n) {
          : n == 60  ? 15
          : n == 72  ? 18
          : n == 80  ? 10
+         : n == 90  ? 15
          : n == 100 ? 25
          : n == 108 ? 27
          : n == 120 ? 15
          : n == 144 ? 18
          : n == 160 ? 10
+         : n == 180 ? 15
          : n == 216 ? 27
                     : (sizeof(T) == 8 ? 4 : 8);  // 256
 }
@@ -452,8 +450,19 @@ struct ZShape {
   static constexpr int LZ = padded_len<T>(NZ);
   // entries per thread when no triplet repeats (S full sticks)
   static constexpr int K = (S * NZ + NT - 1) / NT;
-  // tile, one spare element that absorbs repeated triplets, twiddle table
-  static constexpr int TW = S * LZ + 1;
+  // tile, one spare element that absorbs repeated triplets, the spare lanes'
+  // dummy lines (one per wave), twiddle table
+  static constexpr int DUMMY = S * LZ + 1;
+  static constexpr int TW = DUMMY + (L::SPARE ? (NT / kWave) * LZ : 0);
+  // Stick of this lane within the workgroup (S for a spare lane: none) and
+  // the tile row its line exchanges through.
+  __device__ static int stick(int lane) {
+    return L::spare(lane) ? S : static_cast<int>(threadIdx.x / kWave) * L::PER_WAVE + lane / L::P;
+  }
+  template <typename C>
+  __device__ static C* line(C* tile, int lane) {
+    return L::spare(lane) ? tile + DUMMY + (threadIdx.x / kWave) * LZ : tile + stick(lane) * LZ;
+  }
   // twiddle table after the tile
   static size_t lds() {
     return static_cast<size_t>(tw_offset<T>(TW)) * sizeof(Cx<T>) +
@@ -493,9 +502,9 @@ __device__ __forceinline__ void z_backward_runs(const FastArgs<T>& a, int bid) {
   const int s0 = bid * S;
   const int ns = min(S, a.nst - s0);
   const int lane = threadIdx.x % kWave;
-  const int sl = (threadIdx.x / kWave) * L::PER_WAVE + lane / L::P;
+  const int sl = Z::stick(lane);
   const int q = lane % L::P;
-  const bool live = s0 + sl < a.nst;
+  const bool live = sl < S && s0 + sl < a.nst;
   const int4 run = a.zInfo[live ? s0 + sl : a.nst - 1];
   const Cx<T>* in = reinterpret_cast<const Cx<T>*>(a.in);
   Cx<T> v[L::E];
@@ -508,7 +517,7 @@ __device__ __forceinline__ void z_backward_runs(const FastArgs<T>& a, int bid) {
     v[e] = present ? x : Cx<T>{T(0), T(0)};
   }
   if (threadIdx.x < NZ) twl[threadIdx.x] = a.twZ[threadIdx.x];
-  Cx<T>* line = tile + sl * LZ;
+  Cx<T>* line = Z::line(tile, lane);
   if (RC && s0 == 0 && a.zeroZeroStick == 0) {
 #pragma unroll
     for (int e = 0; e < L::E; ++e) line[phys<T>(q + L::P * e)] = v[e];
@@ -591,9 +600,8 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
     a.zbuf[static_cast<size_t>(threadIdx.x) * a.pitch + a.nst] = Cx<T>{T(0), T(0)};
   }
   const int lane = threadIdx.x % kWave;
-  const int sl = (threadIdx.x / kWave) * L::PER_WAVE + lane / L::P;
   const int q = lane % L::P;
-  Cx<T>* line = tile + sl * LZ;
+  Cx<T>* line = Z::line(tile, lane);
   Cx<T> v[L::E];
 #pragma unroll
   for (int e = 0; e < L::E; ++e) v[e] = line[phys<T>(q + L::P * e)];
@@ -617,10 +625,16 @@ __device__ __forceinline__ void z_forward_body(const FastArgs<T>& a, int bid) {
   Tw<T>* twl = reinterpret_cast<Tw<T>*>(tile + tw_offset<T>(Z::TW));
   const int s0 = bid * S;
   const int ns = min(S, a.nst - s0);
-  {
+  if constexpr (S * NZ != L::E * NT || NT % S != 0) {
+    // spare lanes: no whole-tile thread mapping
+    for (int t = threadIdx.x; t < S * NZ; t += NT) {
+      const int s = t % S, zz = t / S;
+      tile[s * LZ + phys<T>(zz)] = s < ns ? a.zbuf[static_cast<size_t>(zz) * a.pitch + s0 + s]
+                                          : Cx<T>{T(0), T(0)};
+    }
+  } else {
     // thread t holds stick t % S at z = t / S + P k, k < E (S NZ = E NT): all
     // loads in flight before the tile stores
-    static_assert(S * NZ == L::E * NT && NT % S == 0, "tile load shape");
     constexpr int ZS = NT / S;  // z rows per pass of the workgroup
     const int s = threadIdx.x % S, zz0 = threadIdx.x / S;
     const int sc = s < ns ? s : 0;
@@ -639,12 +653,12 @@ __device__ __forceinline__ void z_forward_body(const FastArgs<T>& a, int bid) {
     // sticks stored as runs: each lane writes its outputs straight to the
     // run (see z_backward_runs)
     const int lane = threadIdx.x % kWave;
-    const int sl = (threadIdx.x / kWave) * L::PER_WAVE + lane / L::P;
+    const int sl = Z::stick(lane);
     const int q = lane % L::P;
-    const bool live = s0 + sl < a.nst;
+    const bool live = sl < S && s0 + sl < a.nst;
     const int4 run = a.zInfo[live ? s0 + sl : a.nst - 1];
     __syncthreads();
-    Cx<T>* line = tile + sl * LZ;
+    Cx<T>* line = Z::line(tile, lane);
     Cx<T> v[L::E];
 #pragma unroll
     for (int e = 0; e < L::E; ++e) v[e] = line[phys<T>(q + L::P * e)];
@@ -671,9 +685,8 @@ __device__ __forceinline__ void z_forward_body(const FastArgs<T>& a, int bid) {
   }
   __syncthreads();
   const int lane = threadIdx.x % kWave;
-  const int sl = (threadIdx.x / kWave) * L::PER_WAVE + lane / L::P;
   const int q = lane % L::P;
-  Cx<T>* line = tile + sl * LZ;
+  Cx<T>* line = Z::line(tile, lane);
   Cx<T> v[L::E];
 #pragma unroll
   for (int e = 0; e < L::E; ++e) v[e] = line[phys<T>(q + L::P * e)];
@@ -723,6 +736,7 @@ struct XYShape {
   // quarter (6 elements: 12 would keep 48 gathers per lane in flight)
   using YL = Line<M, elems_per_lane(M) / 2>;
   using YB = Line<M, elems_per_lane(M) / (M % 3 == 0 ? 4 : 2)>;
+  static_assert(!FL::SPARE && !YL::SPARE && !YB::SPARE, "whole lines per wave");
   // backward: A = [m][LU] (column u at offset u, y transforms in column
   // mode), reused as B = [rows][LB] (x transforms)
   static constexpr int ROWS_A = padded_len<T>(M);
@@ -1052,6 +1066,7 @@ struct PlaneShape {
   static constexpr int NW = NT / kWave;
   using XL = Line<N, F3 ? 12 : 4>;
   using YL = Line<N, F3 ? 12 : 8>;
+  static_assert(!XL::SPARE && !YL::SPARE, "whole lines per wave");
   static constexpr int ROWS = padded_len<T>(N);  // the column exchange reaches padded rows
   static constexpr int LB = padded_len<T>(N);    // x line in the scratch region
   static constexpr int XROWS = NW * XL::PER_WAVE;
@@ -1090,7 +1105,7 @@ __device__ __forceinline__ void plane_backward_body(const FastArgs<T>& a, int z)
   using S = PlaneShape<T, N>;
   using XL = typename S::XL;
   using YL = typename S::YL;
-  constexpr int NT = S::NT, NW = S::NW, LB = S::LB;
+  constexpr int NW = S::NW, LB = S::LB;
   const int PU = a.pitchT, ncol = a.ncol;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
   Cx<T>* xs = tile + S::ROWS * PU;
@@ -1153,7 +1168,7 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
   using S = PlaneShape<T, N>;
   using XL = typename S::XL;
   using YL = typename S::YL;
-  constexpr int NT = S::NT, NW = S::NW, LB = S::LB;
+  constexpr int NW = S::NW, LB = S::LB;
   const int PU = a.pitchT, ncol = a.ncol;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(spfft_lds_fast);
   Cx<T>* xs = tile + S::ROWS * PU;
@@ -1239,7 +1254,8 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
 template <typename T, int N, typename L>
 constexpr int split_waves() {
   int nw = N / L::PER_WAVE < 4 ? N / L::PER_WAVE : 4;
-  while (nw > 1 && static_cast<size_t>(nw) * L::PER_WAVE * padded_len<T>(N) * sizeof(Cx<T>) >
+  constexpr int regions = L::PER_WAVE + (L::SPARE ? 1 : 0);  // exchange regions per wave
+  while (nw > 1 && static_cast<size_t>(nw) * regions * padded_len<T>(N) * sizeof(Cx<T>) >
                        64 * 1024) {
     nw /= 2;
   }
@@ -1256,8 +1272,20 @@ struct SplitShape {
   static_assert(NW >= 1 && LINES <= N, "split line shape");
   static constexpr int LB = padded_len<T>(N);  // exchange region of one line
   static constexpr int TP = LINES | 1;         // odd pitch of the [y][LINES] tile
+  // exchange regions: the lines', then one dummy line per wave for its spare
+  // lanes (P not dividing the wave)
+  static constexpr int XL = LINES + (L::SPARE ? NW : 0);
   // the exchange regions, reused as the tile between phases
-  static constexpr int REGION = LINES * LB > N * TP ? LINES * LB : N * TP;
+  static constexpr int REGION = XL * LB > N * TP ? XL * LB : N * TP;
+  // This lane's line in the workgroup (LINES for a spare lane: none) and the
+  // exchange region it runs in.
+  __device__ static int line_of(int wave, int lane) {
+    return L::spare(lane) ? LINES : wave * L::PER_WAVE + lane / L::P;
+  }
+  template <typename C>
+  __device__ static C* xregion(C* region, int wave, int lane) {
+    return region + (L::spare(lane) ? LINES + wave : wave * L::PER_WAVE + lane / L::P) * LB;
+  }
   // rows per workgroup of the row kernels (rows of length N): R2C packs two
   // real rows into each complex line
   __host__ __device__ static constexpr int row_block(bool rc) { return rc ? 2 * LINES : LINES; }
@@ -1295,9 +1323,10 @@ __device__ __forceinline__ void ysplit_backward_body(const FastArgs<T>& a, int b
   Cx<T>* fill = reinterpret_cast<Cx<T>*>(twl + N);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int q = lane % P;
-  const int line = wave * L::PER_WAVE + lane / P;
+  const int line = S::line_of(wave, lane);
   const int u = u0 + line;
-  const int uc = u < a.ncol ? u : a.ncol;  // row ncol of the table: the zero element
+  // row ncol of the table: the zero element (also for spare lanes)
+  const int uc = line < S::LINES && u < a.ncol ? u : a.ncol;
   const Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
   Cx<T> v[E];
 #pragma unroll
@@ -1329,11 +1358,13 @@ __device__ __forceinline__ void ysplit_backward_body(const FastArgs<T>& a, int b
   } else {
     __syncthreads();  // twiddles
   }
-  Cx<T>* lr = region + line * S::LB;
+  Cx<T>* lr = S::xregion(region, wave, lane);
   wave_fft<T, N, true, 1, E>(v, lr, 1, q, twl);
   __syncthreads();  // the exchange regions become the tile
+  if (line < S::LINES) {
 #pragma unroll
-  for (int e = 0; e < E; ++e) region[(q + P * e) * S::TP + line] = v[e];
+    for (int e = 0; e < E; ++e) region[(q + P * e) * S::TP + line] = v[e];
+  }
   __syncthreads();
   Cx<T>* yrows = a.ybuf + static_cast<size_t>(z) * N * a.pitchY + u0;
   const int nc = min(S::LINES, a.ncol - u0);
@@ -1357,10 +1388,10 @@ __device__ __forceinline__ void xsplit_backward_body(const FastArgs<T>& a, int b
   Tw<T>* twl = reinterpret_cast<Tw<T>*>(region + tw_offset<T>(S::REGION));
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int q = lane % P;
-  const int line = wave * L::PER_WAVE + lane / P;
+  const int line = S::line_of(wave, lane);
   // C2C: row y0 + line. R2C: rows y0 + 2 line (real part of the line's
   // result) and y0 + 2 line + 1 (imaginary part); ny is even.
-  const int y = y0 + (RC ? 2 * line : line);
+  const int y = line < S::LINES ? y0 + (RC ? 2 * line : line) : ny;  // spare lanes: no row
   const int yc = y < ny ? y : ny - 1 - (RC ? 1 : 0);
   const int ncol = a.ncol;
   const Cx<T>* yrow = a.ybuf + (static_cast<size_t>(z) * ny + yc) * a.pitchY;
@@ -1392,7 +1423,7 @@ __device__ __forceinline__ void xsplit_backward_body(const FastArgs<T>& a, int b
   }
   for (int t = threadIdx.x; t < N; t += S::NT) twl[t] = a.twX[t];
   __syncthreads();
-  wave_fft<T, N, true, 1, E>(v, region + line * S::LB, 1, q, twl);
+  wave_fft<T, N, true, 1, E>(v, S::xregion(region, wave, lane), 1, q, twl);
   if (y < ny) {
     const size_t row = (static_cast<size_t>(z) * ny + y) * N;
     if (RC) {
@@ -1423,10 +1454,10 @@ __device__ __forceinline__ void xsplit_forward_body(const FastArgs<T>& a, int bi
   Tw<T>* twl = reinterpret_cast<Tw<T>*>(region + tw_offset<T>(S::REGION));
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int q = lane % P;
-  const int line = wave * L::PER_WAVE + lane / P;
+  const int line = S::line_of(wave, lane);
   // C2C: row y0 + line. R2C: the real rows a = y0 + 2 line and b = a + 1
   // enter as a + i b; ny is even.
-  const int y = y0 + (RC ? 2 * line : line);
+  const int y = line < S::LINES ? y0 + (RC ? 2 * line : line) : ny;  // spare lanes: no row
   const int yc = y < ny ? y : ny - 1 - (RC ? 1 : 0);
   const size_t row = (static_cast<size_t>(z) * ny + yc) * N;
   Cx<T> v[E];
@@ -1443,7 +1474,7 @@ __device__ __forceinline__ void xsplit_forward_body(const FastArgs<T>& a, int bi
   for (int e = 0; e < E; ++e) uu[e] = a.xCol[q + P * e];
   for (int t = threadIdx.x; t < N; t += S::NT) twl[t] = a.twX[t];
   __syncthreads();
-  Cx<T>* lr = region + line * S::LB;
+  Cx<T>* lr = S::xregion(region, wave, lane);
   wave_fft<T, N, false, 1, E>(v, lr, 1, q, twl);
   if (RC) {
     // Z = A + i B with A, B the spectra of the real rows a and b:
@@ -1489,15 +1520,21 @@ __device__ __forceinline__ void ysplit_forward_body(const FastArgs<T>& a, int bi
   Tw<T>* twl = reinterpret_cast<Tw<T>*>(region + tw_offset<T>(S::REGION));
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int q = lane % P;
-  const int line = wave * L::PER_WAVE + lane / P;
-  const int u = u0 + line;
+  const int line = S::line_of(wave, lane);
+  const int u = line < S::LINES ? u0 + line : a.ncol;  // spare lanes: no column
   const int nc = min(S::LINES, a.ncol - u0);
   // LINES columns of every row through the tile (row segments); columns past
   // ncol are zero. Thread t holds column t % LINES of rows t / LINES + P k,
   // k < E (N LINES = E NT), all loads in flight before the tile stores.
   const Cx<T>* yrows = a.ybuf + static_cast<size_t>(z) * N * a.pitchY + u0;
-  {
-    static_assert(N * S::LINES == E * S::NT && S::NT % S::LINES == 0, "tile load shape");
+  if constexpr (N * S::LINES != E * S::NT || S::NT % S::LINES != 0) {
+    // spare lanes: no whole-tile thread mapping
+    for (int t = threadIdx.x; t < N * S::LINES; t += S::NT) {
+      const int y = t / S::LINES, c = t - y * S::LINES;
+      region[y * S::TP + c] =
+          c < nc ? yrows[static_cast<size_t>(y) * a.pitchY + c] : Cx<T>{T(0), T(0)};
+    }
+  } else {
     constexpr int RS = S::NT / S::LINES;  // rows per pass of the workgroup
     const int c = threadIdx.x % S::LINES, y0 = threadIdx.x / S::LINES;
     const int cc = c < nc ? c : 0;
@@ -1513,9 +1550,9 @@ __device__ __forceinline__ void ysplit_forward_body(const FastArgs<T>& a, int bi
   __syncthreads();
   Cx<T> v[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = region[(q + P * e) * S::TP + line];
+  for (int e = 0; e < E; ++e) v[e] = region[(q + P * e) * S::TP + (line < S::LINES ? line : 0)];
   __syncthreads();  // the tile becomes the exchange regions
-  wave_fft<T, N, false, 1, E>(v, region + line * S::LB, 1, q, twl);
+  wave_fft<T, N, false, 1, E>(v, S::xregion(region, wave, lane), 1, q, twl);
   if (u < a.ncol) {
     Cx<T>* zrow = a.zbuf + static_cast<size_t>(z) * a.pitch;
 #pragma unroll
@@ -2076,6 +2113,7 @@ bool fast_length(int n) { return square_length(n) || split_only_length(n); }
     case 64: CALL(64); break;      \
     case 72: CALL(72); break;      \
     case 80: CALL(80); break;      \
+    case 90: CALL(90); break;      \
     case 96: CALL(96); break;      \
     case 100: CALL(100); break;    \
     case 108: CALL(108); break;    \
@@ -2083,6 +2121,7 @@ bool fast_length(int n) { return square_length(n) || split_only_length(n); }
     case 128: CALL(128); break;    \
     case 144: CALL(144); break;    \
     case 160: CALL(160); break;    \
+    case 180: CALL(180); break;    \
     case 192: CALL(192); break;    \
     case 216: CALL(216); break;    \
     case 256: CALL(256); break;    \

@@ -71,7 +71,10 @@ struct FastArgs {
 };
 
 // Residue classes per plane used by the xy kernels of length n.
-constexpr int fast_classes(int n) { return n >= 192 ? 8 : 4; }
+// (A divisor of n: the stick index table is class-interleaved with it.)
+constexpr int fast_classes(int n) {
+  return n >= 192 && n % 8 == 0 ? 8 : (n % 4 == 0 ? 4 : (n % 2 == 0 ? 2 : 1));
+}
 
 // Geometry of the z kernels' LDS tile: sticks per workgroup, padded line
 // length and the padding shift of the index map i -> i + (i >> shift).
@@ -87,8 +90,8 @@ inline int fast_pitch(int nst) {
 }
 
 // True when (nx, ny, nz) have fast kernels: z kernels and the split plane
-// stage for every length in {32, 48, 60, 64, 72, 80, 96, 100, 108, 120, 128,
-// 144, 160, 192, 216, 256}.
+// stage for every length in {32, 48, 60, 64, 72, 80, 90, 96, 100, 108, 120,
+// 128, 144, 160, 180, 192, 216, 256}.
 template <typename T>
 bool fast_path_supported(int nx, int ny, int nz, int ncol, bool r2c);
 // True when the class-split plane kernels serve the plane too: nx = ny in

@@ -1,6 +1,7 @@
 """Lengths served by the z kernels and the split plane stage only (48, 72, 80,
-144, 160, 256, and on mixed radix schedules 60, 100, 108, 120, 216: no
-class-split kernels), chosen without forcing, against the FP64 oracle: FP64
+144, 160, 256; on mixed radix schedules 60, 100, 108, 120, 216; with lines of
+6 and 12 lanes, whose waves have spare lanes, 90 and 180: no class-split
+kernels), chosen without forcing, against the FP64 oracle: FP64
 <= 1e-12, FP32 <= 1e-5 relative L2. Cubic grids with
 spheres and random column patterns, C2C and R2C, grids mixing these lengths
 with each other and with the class-split lengths, uneven batches (one fused
@@ -20,9 +21,10 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 TOL = {"double": 1e-12, "float": 1e-5}
-LENGTHS = [48, 60, 72, 80, 100, 108, 120, 144, 160, 216, 256]
+LENGTHS = [48, 60, 72, 80, 90, 100, 108, 120, 144, 160, 180, 216, 256]
 MIXED = [(72, 80, 48), (160, 144, 256), (256, 48, 96), (80, 128, 160), (192, 72, 144),
-         (48, 256, 32), (120, 100, 60), (216, 108, 120), (100, 216, 64)]
+         (48, 256, 32), (120, 100, 60), (216, 108, 120), (100, 216, 64), (90, 180, 96),
+         (180, 60, 90)]
 
 
 def check_single(dims, t, r2c, precision, seed):
@@ -73,7 +75,8 @@ def test_split_length_mixed_grids(dims, r2c, precision):
 
 @pytest.mark.parametrize("precision", ["double", "float"])
 @pytest.mark.parametrize("r2c", [False, True], ids=["c2c", "r2c"])
-@pytest.mark.parametrize("dims", [(144, 144, 144), (80, 72, 160), (120, 108, 100)],
+@pytest.mark.parametrize("dims", [(144, 144, 144), (80, 72, 160), (120, 108, 100),
+                                  (90, 180, 90)],
                          ids=lambda d: "x".join(map(str, d)))
 def test_split_length_uneven_batch(dims, r2c, precision):
     nx, ny, nz = dims
