@@ -766,7 +766,12 @@ struct XYShape {
   static constexpr int ROWS_FWD =
       padded_len<T>(N) > round_up(N, YL::PER_WAVE) ? padded_len<T>(N)
                                                   : round_up(N, YL::PER_WAVE);
-  static size_t lds_forward() { return static_cast<size_t>(ROWS_FWD) * LT * sizeof(Cx<T>); }
+  // tile, then the MX-point twiddle table W_M of the x lines (their FFTs then
+  // load no twiddles from memory, so the next rows' loads can stay in flight)
+  static constexpr int TW_FWD = tw_offset<T>(ROWS_FWD * LT);
+  static size_t lds_forward() {
+    return static_cast<size_t>(TW_FWD) * sizeof(Cx<T>) + static_cast<size_t>(M) * sizeof(Tw<T>);
+  }
 };
 
 template <typename T, int N, bool RC>
@@ -971,8 +976,13 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
 
   // x: every row, decimation in frequency to the columns x = c + R m, then
   // the MX-point FFT; row y's result lands in Tb[y][m]. Class twiddles are
-  // loaded once; the row loads of a lane are issued together.
+  // loaded once; the row loads of a lane are issued together. When a lane's
+  // loads for one row block fit one batch, the next block's loads are issued
+  // before the current block's FFT (whose twiddles come from LDS), so they
+  // overlap it.
   {
+    Tw<T>* tw32 = reinterpret_cast<Tw<T>*>(Tb + S::TW_FWD);
+    for (int t = threadIdx.x; t < MX; t += S::NT) tw32[t] = twl[t * R];
     const int q = lane % LM::P;
     Tw<T> wl[R];
     wl[0] = tw_one<T>();
@@ -981,39 +991,65 @@ __device__ __forceinline__ void xy_forward_body(const FastArgs<T>& a, int bid) {
     Tw<T> wj[LM::E];
 #pragma unroll
     for (int e = 0; e < LM::E; ++e) wj[e] = twiddle<false>(twl, c * (q + LM::P * e));
-    for (int y0 = wave * LM::PER_WAVE; y0 < N; y0 += NW * LM::PER_WAVE) {
+    lds_barrier();  // tw32
+    // at most 16 loads per lane in flight
+    constexpr int EC = (LM::E % 4 == 0 && 4 * R <= 16) ? 4 : (LM::E % 2 == 0 ? 2 : LM::E);
+    constexpr int STRIDE = NW * LM::PER_WAVE;  // rows per round of the workgroup
+    auto load = [&](int y0, Cx<T> (&xv)[EC][R], int e0) {
       const int y = y0 + lane / LM::P;
       const int yc = y < N ? y : N - 1;
-      const Cx<T>* row = in + static_cast<size_t>(yc) * N;
-      const T* rrow = a.in + (static_cast<size_t>(z) * N + yc) * N;
-      Cx<T> v[LM::E];
-      // at most 16 loads per lane in flight
-      constexpr int EC = (LM::E % 4 == 0 && 4 * R <= 16) ? 4 : (LM::E % 2 == 0 ? 2 : LM::E);
-#pragma unroll
-      for (int e0 = 0; e0 < LM::E; e0 += EC) {
-        Cx<T> xv[EC][R];
-        if (RC) {
-#pragma unroll
-          for (int e = 0; e < EC; ++e) {
-#pragma unroll
-            for (int l = 0; l < R; ++l) xv[e][l] = Cx<T>{rrow[q + LM::P * (e0 + e) + MX * l], T(0)};
-          }
-        } else {
-#pragma unroll
-          for (int e = 0; e < EC; ++e) {
-#pragma unroll
-            for (int l = 0; l < R; ++l) xv[e][l] = row[q + LM::P * (e0 + e) + MX * l];
-          }
-        }
+      if (RC) {
+        const T* rrow = a.in + (static_cast<size_t>(z) * N + yc) * N;
 #pragma unroll
         for (int e = 0; e < EC; ++e) {
-          v[e0 + e] = cmul(dif_sum<T, R>(xv[e], wl), wj[e0 + e]);
+#pragma unroll
+          for (int l = 0; l < R; ++l) xv[e][l] = Cx<T>{rrow[q + LM::P * (e0 + e) + MX * l], T(0)};
+        }
+      } else {
+        const Cx<T>* row = in + static_cast<size_t>(yc) * N;
+#pragma unroll
+        for (int e = 0; e < EC; ++e) {
+#pragma unroll
+          for (int l = 0; l < R; ++l) xv[e][l] = row[q + LM::P * (e0 + e) + MX * l];
         }
       }
-      Cx<T>* line = Tb + y * LT;
-      wave_fft<T, MX, false, R, LM::E>(v, line, 1, q, twl);
+    };
+    auto dif = [&](Cx<T> (&xv)[EC][R], Cx<T>* v, int e0) {
+#pragma unroll
+      for (int e = 0; e < EC; ++e) v[e0 + e] = cmul(dif_sum<T, R>(xv[e], wl), wj[e0 + e]);
+    };
+    auto fft_store = [&](int y0, Cx<T>* v) {
+      Cx<T>* line = Tb + (y0 + lane / LM::P) * LT;
+      wave_fft<T, MX, false, 1, LM::E>(v, line, 1, q, tw32);
 #pragma unroll
       for (int e = 0; e < LM::E; ++e) line[phys<T>(q + LM::P * e)] = v[e];
+    };
+    int y0 = wave * LM::PER_WAVE;
+    if constexpr (EC == LM::E) {
+      // one load batch per block: software-pipelined over the wave's blocks
+      if (y0 < N) {
+        Cx<T> xv[EC][R], v[LM::E];
+        load(y0, xv, 0);
+        dif(xv, v, 0);
+        for (int y1 = y0 + STRIDE; y1 < N; y1 += STRIDE) {
+          load(y1, xv, 0);
+          fft_store(y0, v);
+          dif(xv, v, 0);
+          y0 = y1;
+        }
+        fft_store(y0, v);
+      }
+    } else {
+      for (; y0 < N; y0 += STRIDE) {
+        Cx<T> v[LM::E];
+#pragma unroll
+        for (int e0 = 0; e0 < LM::E; e0 += EC) {
+          Cx<T> xv[EC][R];
+          load(y0, xv, e0);
+          dif(xv, v, e0);
+        }
+        fft_store(y0, v);
+      }
     }
   }
   __syncthreads();
