@@ -2544,7 +2544,12 @@ void launch_split_xy_batch(bool forward, bool r2c, int nx, int ny, const FastArg
   template void launch_fast_xy_backward<T>(int, int, const FastArgs<T>&, hipStream_t);   \
   template void launch_fast_xy_forward<T>(int, int, const FastArgs<T>&, hipStream_t);
 
-SPFFT_INSTANTIATE_FAST(double)
+// One precision per translation unit (the two build in parallel):
+// fast_kernels_float.hip includes this file with SPFFT_FAST_FLOAT_UNIT set.
+#ifdef SPFFT_FAST_FLOAT_UNIT
 SPFFT_INSTANTIATE_FAST(float)
+#else
+SPFFT_INSTANTIATE_FAST(double)
+#endif
 
 }  // namespace spfft
