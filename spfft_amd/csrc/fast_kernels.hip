@@ -78,7 +78,7 @@ constexpr bool dividing_schedule(int n, int e) {
 }
 
 // Mixed schedule, for lines whose lane element count cannot take every radix
-// (e = 15 with a factor 8 in n, e = 25 or 27 with a factor 4): the first pass
+// (an odd e = 15, 25 or 27 with factors 2 in n): the first pass
 // reads the lane's own elements and the last pass leaves them in natural
 // order, so their radices divide e; the passes between them read their
 // butterflies' inputs from the line's LDS region anyway, so a lane there
@@ -1283,7 +1283,9 @@ __device__ __forceinline__ void plane_forward_body(const FastArgs<T>& a, int z) 
 //             columns (LINES columns enter through the LDS tile as row
 //             segments, y FFT, results written to the sticks).
 // Lines are FullLine<T, N> (P lanes x E elements); LINES = lines per
-// workgroup. No plane is read more than once.
+// workgroup. No plane is read more than once. R2C: each x line carries two
+// rows (a + i b). Where P does not divide the wave, its spare lanes run a
+// dummy line in an exchange region of their own and store nothing.
 
 // Waves of a split-stage workgroup: up to 4, never more lines than the plane
 // has rows (columns), and the exchange regions within 64 KiB.

@@ -13,11 +13,18 @@
 //       over x on every row, M-point x FFTs, then full-length y FFTs on the
 //       class's columns; results are gathered at the stick positions.
 //   Each class reads the whole plane input (from L2 after the first class);
-//   the classes of a plane are mapped to the same XCD.
+//   the classes of a plane are mapped to the same XCD. Class-split kernels
+//   exist for nx = ny in {32, 64, 96, 128, 192}.
+//   split plane stage (every fixed length, any nx and ny): a y kernel and an
+//   x kernel per direction around a compact column buffer [z][y][column]
+//   held in the grid's work buffer; every plane element is read once.
 //   R2C (a compile-time variant): the backward z kernel fills the (0,0)
-//   stick hermitian, the backward xy kernel fills the x = 0 column of each
-//   plane and completes every row from its half spectrum before a complex x
-//   FFT whose real part is stored; the forward xy kernel reads real rows.
+//   stick hermitian, the backward plane kernels fill the x = 0 column of
+//   each plane and complete every row from its half spectrum; two hermitian
+//   rows a, b enter one complex x FFT as a + i b, whose real and imaginary
+//   parts are the two real rows (the split stage's forward x kernel packs two
+//   real rows the same way and separates their spectra); the forward
+//   class-split kernel reads real rows.
 #pragma once
 
 #include <hip/hip_runtime.h>
