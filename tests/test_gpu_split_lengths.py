@@ -103,21 +103,24 @@ def test_split_length_uneven_batch(dims, r2c, precision):
         assert orc.rel_l2(f, o.forward(got, True)) <= TOL[precision], k
 
 
-def test_split_length_host_pointers():
+@pytest.mark.parametrize("n,r2c", [(160, False), (90, True), (216, False)],
+                         ids=["160-c2c", "90-r2c", "216-c2c"])
+def test_split_length_host_pointers(n, r2c):
     """Host buffers: values staged in the work buffer, planes in chunks."""
-    dims = (160, 160, 160)
+    dims = (n, n, n)
     nx, ny, nz = dims
-    t = sphere_triplets(50)
-    o = orc.OraclePlan(False, dims, t)
-    tr = build(dims, t, False)
-    v = complex_values(0x5EED0D80, len(t))
-    space = np.empty(2 * nx * ny * nz)
+    t = sphere_triplets(0.3 * n, half=r2c)
+    o = orc.OraclePlan(r2c, dims, t)
+    tr = build(dims, t, r2c)
+    v = complex_values(0x5EED0D80 + n, len(t))
+    space = np.empty(nx * ny * nz * (1 if r2c else 2))
     tr.backward(v.view(np.float64).ctypes.data, space.ctypes.data)
     want = o.backward(v)
-    assert orc.rel_l2(space.view(np.complex128).reshape(nz, ny, nx), want) <= 1e-12
+    got = space.reshape(nz, ny, nx) if r2c else space.view(np.complex128).reshape(nz, ny, nx)
+    assert orc.rel_l2(got, want) <= 1e-12
     out = np.empty(2 * len(t))
     tr.forward(space.ctypes.data, out.ctypes.data, sp.SPFFT_FULL_SCALING)
-    assert orc.rel_l2(out.view(np.complex128), o.forward(want, True)) <= 1e-12
+    assert orc.rel_l2(out.view(np.complex128), o.forward(got, True)) <= 1e-12
 
 
 def kernels_run(fn):

@@ -70,6 +70,13 @@ def test_noalloc_single_transform():
     run_pair(96, 30, 1, 0x5EED0720)
 
 
+def test_noalloc_split_stage_lengths():
+    """Lengths served by the split plane stage only: a mixed radix schedule
+    (120), then lines whose waves have spare lanes (90), smaller."""
+    run_pair(120, 37, 4, 0x5EED0740)
+    run_pair(90, 28, 3, 0x5EED0750)
+
+
 def test_noalloc_scratch_limit():
     """64 x 64 x 128 transforms with all 4,096 sticks (one value each): 512
     blocks of 8 sticks, so 224 + 4,096 bytes of tables per transform. 240 fit
