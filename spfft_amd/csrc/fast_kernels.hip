@@ -147,15 +147,25 @@ constexpr bool schedule_ok(int n, int e) {
 // 48: 4 4 3 on 12; 256: 8 8 4 on 8 (z: 4 4 4 4 on 4)).
 // Mixed schedules (see mixed_schedule): 60 = 4 x 15 (5 4 3), 120 = 8 x 15
 // (5 8 3), 100 = 4 x 25 (5 2 2 5), 108 = 4 x 27 (3 3 4 3), 216 = 8 x 27
-// (3 3 4 2 3); with spare lanes (6 and 12 lanes per line): 90 = 6 x 15
-// (5 3 2 3), 180 = 12 x 15 (5 3 4 3).
+// (3 3 4 2 3), 200 = 8 x 25 (5 2 2 2 5), 240 = 16 x 15 (5 8 2 3); with spare
+// lanes (lanes per line not dividing the wave): 90 = 6 x 15 (5 3 2 3), 180 =
+// 12 x 15 (5 3 4 3), 150 = 10 x 15 (5 3 2 5), 162 = 6 x 27 (3 3 3 2 3), and
+// on dividing schedules 50 = 5 x 10, 54 = 3 x 18; 40 = 4 x 10.
 constexpr bool split_only_length(int n) {
-  return n == 48 || n == 60 || n == 72 || n == 80 || n == 90 || n == 100 || n == 108 ||
-         n == 120 || n == 144 || n == 160 || n == 180 || n == 216 || n == 256;
+  return n == 40 || n == 48 || n == 50 || n == 54 || n == 60 || n == 72 || n == 80 ||
+         n == 90 || n == 100 || n == 108 || n == 120 || n == 144 || n == 150 || n == 160 ||
+         n == 162 || n == 180 || n == 200 || n == 216 || n == 240 || n == 256;
 }
 template <typename T>
 constexpr int split_only_elems(int n) {
-  return n == 48    ? 12
+  return n == 40    ? 10
+         : n == 48  ? 12
+         : n == 50  ? 10
+         : n == 54  ? 18
+         : n == 150 ? 15
+         : n == 162 ? 27
+         : n == 200 ? 25
+         : n == 240 ? 15
          : n == 60  ? 15
          : n == 72  ? 18
          : n == 80  ? (sizeof(T) == 8 ? 10 : 20)
@@ -171,7 +181,14 @@ constexpr int split_only_elems(int n) {
 }
 template <typename T>
 constexpr int split_only_z_elems(int n) {
-  return n == 48    ? 12
+  return n == 40    ? 10
+         : n == 48  ? 12
+         : n == 50  ? 10
+         : n == 54  ? 18
+         : n == 150 ? 15
+         : n == 162 ? 27
+         : n == 200 ? 25
+         : n == 240 ? 15
          : n == 60  ? 15
          : n == 72  ? 18
          : n == 80  ? 10
@@ -2146,7 +2163,10 @@ bool fast_length(int n) { return square_length(n) || split_only_length(n); }
 #define SPFFT_FAST_SWITCH(N, CALL) \
   switch (N) {                     \
     case 32: CALL(32); break;      \
+    case 40: CALL(40); break;      \
     case 48: CALL(48); break;      \
+    case 50: CALL(50); break;      \
+    case 54: CALL(54); break;      \
     case 60: CALL(60); break;      \
     case 64: CALL(64); break;      \
     case 72: CALL(72); break;      \
@@ -2158,10 +2178,14 @@ bool fast_length(int n) { return square_length(n) || split_only_length(n); }
     case 120: CALL(120); break;    \
     case 128: CALL(128); break;    \
     case 144: CALL(144); break;    \
+    case 150: CALL(150); break;    \
     case 160: CALL(160); break;    \
+    case 162: CALL(162); break;    \
     case 180: CALL(180); break;    \
     case 192: CALL(192); break;    \
+    case 200: CALL(200); break;    \
     case 216: CALL(216); break;    \
+    case 240: CALL(240); break;    \
     case 256: CALL(256); break;    \
     default: throw GPUFFTError();  \
   }

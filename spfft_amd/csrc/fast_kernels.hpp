@@ -97,8 +97,8 @@ inline int fast_pitch(int nst) {
 }
 
 // True when (nx, ny, nz) have fast kernels: z kernels and the split plane
-// stage for every length in {32, 48, 60, 64, 72, 80, 90, 96, 100, 108, 120,
-// 128, 144, 160, 180, 192, 216, 256}.
+// stage for every length in {32, 40, 48, 50, 54, 60, 64, 72, 80, 90, 96, 100,
+// 108, 120, 128, 144, 150, 160, 162, 180, 192, 200, 216, 240, 256}.
 template <typename T>
 bool fast_path_supported(int nx, int ny, int nz, int ncol, bool r2c);
 // True when the class-split plane kernels serve the plane too: nx = ny in

@@ -1,7 +1,8 @@
-"""Lengths served by the z kernels and the split plane stage only (48, 72, 80,
-144, 160, 256; on mixed radix schedules 60, 100, 108, 120, 216; with lines of
-6 and 12 lanes, whose waves have spare lanes, 90 and 180: no class-split
-kernels), chosen without forcing, against the FP64 oracle: FP64
+"""Lengths served by the z kernels and the split plane stage only (40, 48, 72,
+80, 144, 160, 256; on mixed radix schedules 60, 100, 108, 120, 200, 216, 240;
+with lines whose lane count does not divide the wave, so that waves have
+spare lanes, 50, 54, 90, 150, 162 and 180: no class-split kernels), chosen
+without forcing, against the FP64 oracle: FP64
 <= 1e-12, FP32 <= 1e-5 relative L2. Cubic grids with
 spheres and random column patterns, C2C and R2C, grids mixing these lengths
 with each other and with the class-split lengths, uneven batches (one fused
@@ -21,10 +22,11 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 TOL = {"double": 1e-12, "float": 1e-5}
-LENGTHS = [48, 60, 72, 80, 90, 100, 108, 120, 144, 160, 180, 216, 256]
+LENGTHS = [40, 48, 50, 54, 60, 72, 80, 90, 100, 108, 120, 144, 150, 160, 162, 180, 200, 216,
+           240, 256]
 MIXED = [(72, 80, 48), (160, 144, 256), (256, 48, 96), (80, 128, 160), (192, 72, 144),
          (48, 256, 32), (120, 100, 60), (216, 108, 120), (100, 216, 64), (90, 180, 96),
-         (180, 60, 90)]
+         (180, 60, 90), (240, 50, 162), (54, 200, 150), (40, 54, 240)]
 
 
 def check_single(dims, t, r2c, precision, seed):
