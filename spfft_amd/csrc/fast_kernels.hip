@@ -61,6 +61,7 @@ constexpr int wave_radix(int m, int e) {
          : (m % 2 == 0 && e % 2 == 0) ? 2
          : (m % 3 == 0 && e % 3 == 0) ? 3
          : (m % 5 == 0 && e % 5 == 0) ? 5
+         : (m % 7 == 0 && e % 7 == 0) ? 7
                                       : -1;
 }
 // True when every pass of the wave-level schedule of an n-point line with e
@@ -91,7 +92,8 @@ struct MixedSchedule {
   bool ok = false;
 };
 constexpr int dividing_pick(int m, int e) {
-  return (m % 5 == 0 && e % 5 == 0)   ? 5
+  return (m % 7 == 0 && e % 7 == 0)   ? 7
+         : (m % 5 == 0 && e % 5 == 0) ? 5
          : (m % 3 == 0 && e % 3 == 0) ? 3
          : (m % 8 == 0 && e % 8 == 0) ? 8
          : (m % 4 == 0 && e % 4 == 0) ? 4
@@ -108,10 +110,10 @@ constexpr MixedSchedule mixed_schedule(int n, int e) {
   if (last < 0) return s;
   s.radix[s.count++] = first;
   int m = n / (first * last);
-  const int cand[5] = {8, 4, 2, 3, 5};
+  const int cand[6] = {8, 4, 2, 3, 5, 7};
   while (m > 1) {
     int best = -1, bestCost = 1 << 30;
-    for (int i = 0; i < 5; ++i) {
+    for (int i = 0; i < 6; ++i) {
       const int r = cand[i];
       if (m % r != 0) continue;
       const int cost = r * ((n / r + p - 1) / p);
@@ -150,15 +152,25 @@ constexpr bool schedule_ok(int n, int e) {
 // (3 3 4 2 3), 200 = 8 x 25 (5 2 2 2 5), 240 = 16 x 15 (5 8 2 3); with spare
 // lanes (lanes per line not dividing the wave): 90 = 6 x 15 (5 3 2 3), 180 =
 // 12 x 15 (5 3 4 3), 150 = 10 x 15 (5 3 2 5), 162 = 6 x 27 (3 3 3 2 3), and
-// on dividing schedules 50 = 5 x 10, 54 = 3 x 18; 40 = 4 x 10.
+// on dividing schedules 50 = 5 x 10, 54 = 3 x 18; 40 = 4 x 10. With a factor
+// 7 (radix-7 passes from the line's twiddle table): 56, 112, 224 = 4, 8, 16 x
+// 14 and 98 = 7 x 14, 196 = 7 x 28 (dividing); 84, 126, 168, 252 = 4, 6, 8, 12
+// x 21 (mixed).
 constexpr bool split_only_length(int n) {
-  return n == 40 || n == 48 || n == 50 || n == 54 || n == 60 || n == 72 || n == 80 ||
-         n == 90 || n == 100 || n == 108 || n == 120 || n == 144 || n == 150 || n == 160 ||
-         n == 162 || n == 180 || n == 200 || n == 216 || n == 240 || n == 256;
+  return n == 40 || n == 48 || n == 50 || n == 54 || n == 56 || n == 60 || n == 72 ||
+         n == 80 || n == 84 || n == 90 || n == 98 || n == 100 || n == 108 || n == 112 ||
+         n == 120 || n == 126 || n == 144 || n == 150 || n == 160 || n == 162 || n == 168 ||
+         n == 180 || n == 196 || n == 200 || n == 216 || n == 224 || n == 240 || n == 252 ||
+         n == 256;
+}
+// Elements per lane of the lengths with a factor 7 (full and z lines alike).
+constexpr int seven_elems(int n) {
+  return n == 84 || n == 126 || n == 168 || n == 252 ? 21 : (n == 196 ? 28 : 14);
 }
 template <typename T>
 constexpr int split_only_elems(int n) {
-  return n == 40    ? 10
+  return n % 7 == 0 ? seven_elems(n)
+         : n == 40  ? 10
          : n == 48  ? 12
          : n == 50  ? 10
          : n == 54  ? 18
@@ -181,7 +193,8 @@ constexpr int split_only_elems(int n) {
 }
 template <typename T>
 constexpr int split_only_z_elems(int n) {
-  return n == 40    ? 10
+  return n % 7 == 0 ? seven_elems(n)
+         : n == 40  ? 10
          : n == 48  ? 12
          : n == 50  ? 10
          : n == 54  ? 18
@@ -318,6 +331,7 @@ __device__ __forceinline__ void wave_fft_mixed(Cx<T>* v, Cx<T>* line, int es, in
   static_assert(S.ok, "no mixed schedule");
   constexpr int PL = N / E;
   constexpr int R = S.radix[PASS];
+  static_assert(R != 7 || TWS == 1, "table butterflies need the line's own table");
   constexpr int NB = N / R;  // butterflies of the pass
   constexpr bool FIRST = PASS == 0, LAST = PASS == S.count - 1;
   static_assert(!(FIRST || LAST) || E % R == 0, "first and last radices divide E");
@@ -377,6 +391,8 @@ __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
   constexpr int PL = N / E;
   constexpr int R = PE > 0 ? full_radix(N / NS, E) : wave_radix(N / NS, E);
   static_assert(R > 0, "no radix divides the lane's element count");
+  // radices without a constant butterfly (7) read W_R from the line's table
+  static_assert(R != 7 || TWS == 1, "table butterflies need the line's own table");
   constexpr int EB = E / R;
   constexpr bool LAST = NS * R == N;
   constexpr int STEP = N / (NS * R) * TWS;
@@ -2167,25 +2183,34 @@ bool fast_length(int n) { return square_length(n) || split_only_length(n); }
     case 48: CALL(48); break;      \
     case 50: CALL(50); break;      \
     case 54: CALL(54); break;      \
+    case 56: CALL(56); break;      \
     case 60: CALL(60); break;      \
     case 64: CALL(64); break;      \
     case 72: CALL(72); break;      \
     case 80: CALL(80); break;      \
+    case 84: CALL(84); break;      \
     case 90: CALL(90); break;      \
     case 96: CALL(96); break;      \
+    case 98: CALL(98); break;      \
     case 100: CALL(100); break;    \
     case 108: CALL(108); break;    \
+    case 112: CALL(112); break;    \
     case 120: CALL(120); break;    \
+    case 126: CALL(126); break;    \
     case 128: CALL(128); break;    \
     case 144: CALL(144); break;    \
     case 150: CALL(150); break;    \
     case 160: CALL(160); break;    \
     case 162: CALL(162); break;    \
+    case 168: CALL(168); break;    \
     case 180: CALL(180); break;    \
     case 192: CALL(192); break;    \
+    case 196: CALL(196); break;    \
     case 200: CALL(200); break;    \
     case 216: CALL(216); break;    \
+    case 224: CALL(224); break;    \
     case 240: CALL(240); break;    \
+    case 252: CALL(252); break;    \
     case 256: CALL(256); break;    \
     default: throw GPUFFTError();  \
   }
@@ -2547,35 +2572,52 @@ void launch_split_xy_batch(bool forward, bool r2c, int nx, int ny, const FastArg
   }
 }
 
-#define SPFFT_INSTANTIATE_FAST(T)                                                        \
+// z kernels
+#define SPFFT_INSTANTIATE_FAST_Z(T)                                                      \
+  template void fast_z_geometry<T>(int, int*, int*, int*);                               \
+  template void launch_fast_z_backward<T>(int, const FastArgs<T>&, hipStream_t);         \
+  template void launch_fast_z_forward<T>(int, const FastArgs<T>&, hipStream_t);          \
+  template void launch_fast_z_batch<T>(bool, bool, int, const FastArgs<T>*, const int2*, \
+                                       int, bool, hipStream_t);
+// split plane stage
+#define SPFFT_INSTANTIATE_FAST_SPLIT(T)                                                  \
+  template int split_pitch<T>(int, int);                                                 \
+  template void launch_split_xy<T>(bool, int, int, const FastArgs<T>&, hipStream_t);     \
+  template void launch_split_xy_batch<T>(bool, bool, int, int, const FastArgs<T>*, int,  \
+                                         int, int, hipStream_t);
+// class-split and whole-plane kernels, path queries
+#define SPFFT_INSTANTIATE_FAST_PLANE(T)                                                  \
   template bool plane_path_supported<T>(int, int, bool);                                 \
   template int plane_pitch<T>(int, int);                                                 \
   template void launch_plane_xy_batch<T>(bool, int, const FastArgs<T>*, int, int, int,   \
                                          int, hipStream_t);                              \
   template int plane_xy_slots<T>(bool, int);                                             \
-  template int split_pitch<T>(int, int);                                                 \
-  template void launch_split_xy<T>(bool, int, int, const FastArgs<T>&, hipStream_t);     \
-  template void launch_split_xy_batch<T>(bool, bool, int, int, const FastArgs<T>*, int,  \
-                                         int, int, hipStream_t);                         \
   template int fast_xy_pitch_a<T>(int, int);                                             \
-  template void launch_fast_z_batch<T>(bool, bool, int, const FastArgs<T>*, const int2*, \
-                                       int, bool, hipStream_t);                          \
   template void launch_fast_xy_batch<T>(bool, bool, int, const FastArgs<T>*, int, int,   \
                                         int, hipStream_t);                               \
   template bool fast_path_supported<T>(int, int, int, int, bool);                        \
   template bool class_split_supported<T>(int, int, int, bool);                           \
-  template void fast_z_geometry<T>(int, int*, int*, int*);                               \
-  template void launch_fast_z_backward<T>(int, const FastArgs<T>&, hipStream_t);         \
-  template void launch_fast_z_forward<T>(int, const FastArgs<T>&, hipStream_t);          \
   template void launch_fast_xy_backward<T>(int, int, const FastArgs<T>&, hipStream_t);   \
   template void launch_fast_xy_forward<T>(int, int, const FastArgs<T>&, hipStream_t);
 
-// One precision per translation unit (the two build in parallel):
-// fast_kernels_float.hip includes this file with SPFFT_FAST_FLOAT_UNIT set.
-#ifdef SPFFT_FAST_FLOAT_UNIT
-SPFFT_INSTANTIATE_FAST(float)
+// One precision and one kernel family per translation unit, so that they
+// build in parallel: the fast_kernels_*.hip files include this file with
+// SPFFT_FAST_UNIT set (1 to 5); compiled alone it is unit 0.
+#ifndef SPFFT_FAST_UNIT
+#define SPFFT_FAST_UNIT 0
+#endif
+#if SPFFT_FAST_UNIT == 0
+SPFFT_INSTANTIATE_FAST_PLANE(double)
+#elif SPFFT_FAST_UNIT == 1
+SPFFT_INSTANTIATE_FAST_Z(double)
+#elif SPFFT_FAST_UNIT == 2
+SPFFT_INSTANTIATE_FAST_SPLIT(double)
+#elif SPFFT_FAST_UNIT == 3
+SPFFT_INSTANTIATE_FAST_PLANE(float)
+#elif SPFFT_FAST_UNIT == 4
+SPFFT_INSTANTIATE_FAST_Z(float)
 #else
-SPFFT_INSTANTIATE_FAST(double)
+SPFFT_INSTANTIATE_FAST_SPLIT(float)
 #endif
 
 }  // namespace spfft

@@ -97,8 +97,9 @@ inline int fast_pitch(int nst) {
 }
 
 // True when (nx, ny, nz) have fast kernels: z kernels and the split plane
-// stage for every length in {32, 40, 48, 50, 54, 60, 64, 72, 80, 90, 96, 100,
-// 108, 120, 128, 144, 150, 160, 162, 180, 192, 200, 216, 240, 256}.
+// stage for every length in {32, 40, 48, 50, 54, 56, 60, 64, 72, 80, 84, 90,
+// 96, 98, 100, 108, 112, 120, 126, 128, 144, 150, 160, 162, 168, 180, 192,
+// 196, 200, 216, 224, 240, 252, 256}.
 template <typename T>
 bool fast_path_supported(int nx, int ny, int nz, int ncol, bool r2c);
 // True when the class-split plane kernels serve the plane too: nx = ny in

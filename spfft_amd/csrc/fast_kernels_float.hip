@@ -1,5 +1,5 @@
-// Single-precision instantiations of the fixed-length kernels: the source of
-// fast_kernels.hip compiled as a second translation unit, so that the two
-// precisions build in parallel.
-#define SPFFT_FAST_FLOAT_UNIT
+// Single-precision class-split and whole-plane kernels of the fixed-length
+// path: the source of fast_kernels.hip compiled as a translation unit of its
+// own, so that the units build in parallel.
+#define SPFFT_FAST_UNIT 3
 #include "fast_kernels.hip"

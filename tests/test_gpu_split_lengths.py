@@ -1,7 +1,8 @@
 """Lengths served by the z kernels and the split plane stage only (40, 48, 72,
 80, 144, 160, 256; on mixed radix schedules 60, 100, 108, 120, 200, 216, 240;
 with lines whose lane count does not divide the wave, so that waves have
-spare lanes, 50, 54, 90, 150, 162 and 180: no class-split kernels), chosen
+spare lanes, 50, 54, 90, 150, 162 and 180; with radix-7 passes 56, 84, 98, 112,
+126, 168, 196, 224 and 252: no class-split kernels), chosen
 without forcing, against the FP64 oracle: FP64
 <= 1e-12, FP32 <= 1e-5 relative L2. Cubic grids with
 spheres and random column patterns, C2C and R2C, grids mixing these lengths
@@ -22,11 +23,12 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 TOL = {"double": 1e-12, "float": 1e-5}
-LENGTHS = [40, 48, 50, 54, 60, 72, 80, 90, 100, 108, 120, 144, 150, 160, 162, 180, 200, 216,
-           240, 256]
+LENGTHS = [40, 48, 50, 54, 56, 60, 72, 80, 84, 90, 98, 100, 108, 112, 120, 126, 144, 150, 160,
+           162, 168, 180, 196, 200, 216, 224, 240, 252, 256]
 MIXED = [(72, 80, 48), (160, 144, 256), (256, 48, 96), (80, 128, 160), (192, 72, 144),
          (48, 256, 32), (120, 100, 60), (216, 108, 120), (100, 216, 64), (90, 180, 96),
-         (180, 60, 90), (240, 50, 162), (54, 200, 150), (40, 54, 240)]
+         (180, 60, 90), (240, 50, 162), (54, 200, 150), (40, 54, 240), (112, 84, 126),
+         (196, 56, 98), (224, 252, 168)]
 
 
 def check_single(dims, t, r2c, precision, seed):
