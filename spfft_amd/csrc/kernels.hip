@@ -1,8 +1,8 @@
 // HIP kernels of the sparse 3D FFT for CDNA4 (gfx950), generic-length path.
 //
 // Every stage kernel stages a batch of 1D lines in LDS, runs a mixed-radix
-// Stockham FFT over them (radix 2/3/4/5/7/8 butterflies in registers, any
-// other prime as a direct DFT from a twiddle table) and writes the result
+// Stockham FFT over them (radix 2/3/4/5/7/8/11/13 butterflies in registers,
+// any other prime as a direct DFT from a twiddle table) and writes the result
 // with the next stage's layout, so the transpose between stages happens at
 // the LDS -> HBM boundary. Twiddles come from one table per length holding
 // exp(-2 pi i t / n) generated on the GPU in FP64; the backward direction uses
@@ -101,6 +101,9 @@ __device__ Cx<T>* lds_fft(Cx<T>* a, Cx<T>* b, const FftDesc& f, int batch,
       case 5: stockham_pass<T, 5, BWD>(a, b, f.n, Ns, batch, w); break;
       case 7: stockham_pass<T, 7, BWD>(a, b, f.n, Ns, batch, w); break;
       case 8: stockham_pass<T, 8, BWD>(a, b, f.n, Ns, batch, w); break;
+      // 11 and 13: the DFT of the radix from the length-n table, unrolled
+      case 11: stockham_pass<T, 11, BWD>(a, b, f.n, Ns, batch, w); break;
+      case 13: stockham_pass<T, 13, BWD>(a, b, f.n, Ns, batch, w); break;
       default: stockham_pass_any<T, BWD>(a, b, f.n, R, Ns, batch, w); break;
     }
     __syncthreads();
