@@ -26,6 +26,41 @@ namespace spfft {
 
 extern __shared__ __attribute__((aligned(16))) unsigned char spfft_lds_fast[];
 
+// One call into the z kernels or the split plane stage of one length. Those
+// kernels are compiled in two groups of lengths (two translation units per
+// precision and family, so that they build in parallel); each group's
+// dispatcher launches the call and returns true when it holds the length.
+template <typename T>
+struct ZCall {
+  enum Op { Backward, Forward, Batch, Geometry } op = Backward;
+  const FastArgs<T>* args = nullptr;  // the transform, or the batch's table
+  const int2* map = nullptr;          // batches: (transform, block) of each block
+  int blocks = 0;
+  int useRuns = 0;
+  bool forward = false;  // batches
+  bool rc = false;       // batches
+  int* geom = nullptr;   // Geometry: sticks per block, line length, pad shift
+  hipStream_t stream = nullptr;
+};
+template <typename T>
+struct SplitCall {
+  bool y = false;  // the y kernel (columns) or the x kernel (rows)
+  bool forward = false;
+  bool batch = false;
+  bool rc = false;                    // batches (single transforms: FastArgs::r2c)
+  const FastArgs<T>* args = nullptr;  // the transform, or the batch's table
+  int ntrans = 0, ny = 0, nz = 0, maxNcol = 0;  // batches
+  hipStream_t stream = nullptr;
+};
+bool z_group0(int n, const ZCall<double>& c);
+bool z_group0(int n, const ZCall<float>& c);
+bool z_group1(int n, const ZCall<double>& c);
+bool z_group1(int n, const ZCall<float>& c);
+bool split_group0(int n, const SplitCall<double>& c);
+bool split_group0(int n, const SplitCall<float>& c);
+bool split_group1(int n, const SplitCall<double>& c);
+bool split_group1(int n, const SplitCall<float>& c);
+
 namespace {
 
 using namespace dev;
@@ -143,77 +178,98 @@ constexpr bool schedule_ok(int n, int e) {
 }
 
 // Lengths served by the split plane stage only (no class-split or whole-plane
-// kernels): elements per lane of their full-length lines and of their z
-// lines, chosen so that every radix divides the lane's elements (72, 144:
-// 2 2 2 (2) 3 3 on 18; 80, 160: 2 2 2 2 (2) 5 on 10, or 4 4 (2) 5 on 20;
-// 48: 4 4 3 on 12; 256: 8 8 4 on 8 (z: 4 4 4 4 on 4)).
-// Mixed schedules (see mixed_schedule): 60 = 4 x 15 (5 4 3), 120 = 8 x 15
-// (5 8 3), 100 = 4 x 25 (5 2 2 5), 108 = 4 x 27 (3 3 4 3), 216 = 8 x 27
-// (3 3 4 2 3), 200 = 8 x 25 (5 2 2 2 5), 240 = 16 x 15 (5 8 2 3); with spare
-// lanes (lanes per line not dividing the wave): 90 = 6 x 15 (5 3 2 3), 180 =
-// 12 x 15 (5 3 4 3), 150 = 10 x 15 (5 3 2 5), 162 = 6 x 27 (3 3 3 2 3), and
-// on dividing schedules 50 = 5 x 10, 54 = 3 x 18; 40 = 4 x 10. With a factor
-// 7 (radix-7 passes from the line's twiddle table): 56, 112, 224 = 4, 8, 16 x
-// 14 and 98 = 7 x 14, 196 = 7 x 28 (dividing); 84, 126, 168, 252 = 4, 6, 8, 12
-// x 21 (mixed).
-constexpr bool split_only_length(int n) {
-  return n == 40 || n == 48 || n == 50 || n == 54 || n == 56 || n == 60 || n == 72 ||
-         n == 80 || n == 84 || n == 90 || n == 98 || n == 100 || n == 108 || n == 112 ||
-         n == 120 || n == 126 || n == 144 || n == 150 || n == 160 || n == 162 || n == 168 ||
-         n == 180 || n == 196 || n == 200 || n == 216 || n == 224 || n == 240 || n == 252 ||
-         n == 256;
+// kernels): elements per lane of their full-length lines and of their z lines
+// (FP64, FP32), chosen so that each line's schedule exists (schedule_ok):
+// dividing schedules, every radix dividing the lane's elements (72, 144: 2 2
+// 2 (2) 3 3 on 18; 80, 160: 2 2 2 2 (2) 5 on 10, or 4 4 (2) 5 on 20; 48: 4 4
+// 3 on 12; 256: 8 8 4 on 8, z 4 4 4 4 on 4; 320: 2 2 2 2 2 2 5 on 10, or 4 4 4
+// 5 on 20; 384: 4 4 4 2 3 on 12, or 8 8 2 3 on 24; 512: 8 8 8 on 8; 36: 2 2 3
+// 3 on 18; 288: 2 2 2 2 2 3 3 on 18), or mixed schedules (mixed_schedule: 60
+// = 4 x 15 (5 4 3), 120 = 8 x 15 (5 8 3), 100 = 4 x 25 (5 2 2 5), ...).
+// Lanes per line not dividing the wave leave spare lanes (50 = 5 x 10, 90 = 6
+// x 15, 45 = 3 x 15, ...). Lengths with a factor 7 take radix-7 passes from
+// the line's twiddle table (56 = 4 x 14: 7 2 2 2, 84 = 4 x 21: 7 2 2 3, ...).
+// Odd lengths (45, 63, 75, 81, 105, 125, 135, 147, 175, 189, 225, 243) run
+// on the same lines; an R2C plane with an odd row count leaves the last
+// packed row pair with one row.
+struct LaneElems {
+  int full64, full32, z64, z32;
+};
+constexpr LaneElems lane_elems(int n) {
+  switch (n) {
+    case 36: return {18, 18, 18, 18};
+    case 40: return {10, 10, 10, 10};
+    case 42: return {21, 21, 21, 21};
+    case 45: return {15, 15, 15, 15};
+    case 48: return {12, 12, 12, 12};
+    case 50: return {10, 10, 10, 10};
+    case 54: return {18, 18, 18, 18};
+    case 56: return {14, 14, 14, 14};
+    case 60: return {15, 15, 15, 15};
+    case 63: return {21, 21, 21, 21};
+    case 70: return {14, 14, 14, 14};
+    case 72: return {18, 18, 18, 18};
+    case 75: return {15, 15, 15, 15};
+    case 80: return {10, 20, 10, 10};
+    case 81: return {27, 27, 27, 27};
+    case 84: return {21, 21, 21, 21};
+    case 90: return {15, 15, 15, 15};
+    case 98: return {14, 14, 14, 14};
+    case 100: return {25, 25, 25, 25};
+    case 105: return {15, 15, 15, 15};
+    case 108: return {27, 27, 27, 27};
+    case 112: return {14, 14, 14, 14};
+    case 120: return {15, 15, 15, 15};
+    case 125: return {25, 25, 25, 25};
+    case 126: return {21, 21, 21, 21};
+    case 135: return {15, 15, 15, 15};
+    case 140: return {14, 14, 14, 14};
+    case 144: return {18, 18, 18, 18};
+    case 147: return {21, 21, 21, 21};
+    case 150: return {15, 15, 15, 15};
+    case 160: return {10, 20, 10, 10};
+    case 162: return {27, 27, 27, 27};
+    case 168: return {21, 21, 21, 21};
+    case 175: return {25, 25, 25, 25};
+    case 180: return {15, 15, 15, 15};
+    case 189: return {21, 21, 21, 21};
+    case 196: return {28, 28, 28, 28};
+    case 200: return {25, 25, 25, 25};
+    case 210: return {21, 21, 21, 21};
+    case 216: return {27, 27, 27, 27};
+    case 224: return {14, 14, 14, 14};
+    case 225: return {15, 15, 15, 15};
+    case 240: return {15, 15, 15, 15};
+    case 243: return {27, 27, 27, 27};
+    case 252: return {21, 21, 21, 21};
+    case 256: return {8, 8, 4, 8};
+    case 270: return {27, 27, 27, 27};
+    case 280: return {14, 14, 14, 14};
+    case 288: return {18, 18, 18, 18};
+    case 300: return {15, 15, 15, 15};
+    case 320: return {10, 20, 10, 10};
+    case 336: return {21, 21, 21, 21};
+    case 360: return {18, 18, 18, 18};
+    case 384: return {12, 24, 12, 12};
+    case 400: return {25, 25, 25, 25};
+    case 420: return {21, 21, 21, 21};
+    case 432: return {27, 27, 27, 27};
+    case 448: return {14, 14, 14, 14};
+    case 450: return {15, 15, 15, 15};
+    case 480: return {15, 15, 15, 15};
+    case 500: return {25, 25, 25, 25};
+    case 512: return {8, 8, 8, 8};
+    default: return {0, 0, 0, 0};
+  }
 }
-// Elements per lane of the lengths with a factor 7 (full and z lines alike).
-constexpr int seven_elems(int n) {
-  return n == 84 || n == 126 || n == 168 || n == 252 ? 21 : (n == 196 ? 28 : 14);
-}
+constexpr bool split_only_length(int n) { return lane_elems(n).full64 > 0; }
 template <typename T>
 constexpr int split_only_elems(int n) {
-  return n % 7 == 0 ? seven_elems(n)
-         : n == 40  ? 10
-         : n == 48  ? 12
-         : n == 50  ? 10
-         : n == 54  ? 18
-         : n == 150 ? 15
-         : n == 162 ? 27
-         : n == 200 ? 25
-         : n == 240 ? 15
-         : n == 60  ? 15
-         : n == 72  ? 18
-         : n == 80  ? (sizeof(T) == 8 ? 10 : 20)
-         : n == 90  ? 15
-         : n == 100 ? 25
-         : n == 108 ? 27
-         : n == 120 ? 15
-         : n == 144 ? 18
-         : n == 160 ? (sizeof(T) == 8 ? 10 : 20)
-         : n == 180 ? 15
-         : n == 216 ? 27
-                    : 8;  // 256
+  return sizeof(T) == 8 ? lane_elems(n).full64 : lane_elems(n).full32;
 }
 template <typename T>
 constexpr int split_only_z_elems(int n) {
-  return n % 7 == 0 ? seven_elems(n)
-         : n == 40  ? 10
-         : n == 48  ? 12
-         : n == 50  ? 10
-         : n == 54  ? 18
-         : n == 150 ? 15
-         : n == 162 ? 27
-         : n == 200 ? 25
-         : n == 240 ? 15
-         : n == 60  ? 15
-         : n == 72  ? 18
-         : n == 80  ? 10
-         : n == 90  ? 15
-         : n == 100 ? 25
-         : n == 108 ? 27
-         : n == 120 ? 15
-         : n == 144 ? 18
-         : n == 160 ? 10
-         : n == 180 ? 15
-         : n == 216 ? 27
-                    : (sizeof(T) == 8 ? 4 : 8);  // 256
+  return sizeof(T) == 8 ? lane_elems(n).z64 : lane_elems(n).z32;
 }
 
 // Lines of the full-tile kernels take the whole lane's elements in their
@@ -549,7 +605,7 @@ __device__ __forceinline__ void z_backward_runs(const FastArgs<T>& a, int bid) {
     const Cx<T> x = in[present ? run.x + pos : run.x];
     v[e] = present ? x : Cx<T>{T(0), T(0)};
   }
-  if (threadIdx.x < NZ) twl[threadIdx.x] = a.twZ[threadIdx.x];
+  for (int t = threadIdx.x; t < NZ; t += NT) twl[t] = a.twZ[t];
   Cx<T>* line = Z::line(tile, lane);
   if (RC && s0 == 0 && a.zeroZeroStick == 0) {
 #pragma unroll
@@ -564,8 +620,10 @@ __device__ __forceinline__ void z_backward_runs(const FastArgs<T>& a, int bid) {
     __syncthreads();  // twiddles
   }
   // the xy kernels read element nst of every row as zero
-  if (bid == 0 && threadIdx.x < NZ) {
-    a.zbuf[static_cast<size_t>(threadIdx.x) * a.pitch + a.nst] = Cx<T>{T(0), T(0)};
+  if (bid == 0) {
+    for (int t = threadIdx.x; t < NZ; t += NT) {
+      a.zbuf[static_cast<size_t>(t) * a.pitch + a.nst] = Cx<T>{T(0), T(0)};
+    }
   }
   wave_fft<T, NZ, true, 1, L::E>(v, line, 1, q, twl);
 #pragma unroll
@@ -604,8 +662,14 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
     ev[k] = x.x;
     ep[k] = e < e1 && x.y >= 0 ? x.y : S * LZ;
   }
-  static_assert(NZ <= NT, "one twiddle per thread");
-  const Tw<T> tw = a.twZ[threadIdx.x < NZ ? threadIdx.x : 0];
+  // twiddles in flight while the tile is cleared (TWK per thread)
+  constexpr int TWK = (NZ + NT - 1) / NT;
+  Tw<T> tw[TWK];
+#pragma unroll
+  for (int k = 0; k < TWK; ++k) {
+    const int t = threadIdx.x + k * NT;
+    tw[k] = a.twZ[t < NZ ? t : 0];
+  }
   for (int t = threadIdx.x; t < S * LZ; t += NT) tile[t] = Cx<T>{T(0), T(0)};
   T vr[K], vi[K];
 #pragma unroll
@@ -614,7 +678,11 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
     vr[k] = x.re;
     vi[k] = x.im;
   }
-  if (threadIdx.x < NZ) twl[threadIdx.x] = tw;
+#pragma unroll
+  for (int k = 0; k < TWK; ++k) {
+    const int t = threadIdx.x + k * NT;
+    if (t < NZ) twl[t] = tw[k];
+  }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < K; ++k) tile[ep[k]] = Cx<T>{vr[k], vi[k]};
@@ -629,8 +697,10 @@ __device__ __forceinline__ void z_backward_body(const FastArgs<T>& a, int bid) {
     __syncthreads();
   }
   // the xy kernels read element nst of every row as zero
-  if (bid == 0 && threadIdx.x < NZ) {
-    a.zbuf[static_cast<size_t>(threadIdx.x) * a.pitch + a.nst] = Cx<T>{T(0), T(0)};
+  if (bid == 0) {
+    for (int t = threadIdx.x; t < NZ; t += NT) {
+      a.zbuf[static_cast<size_t>(t) * a.pitch + a.nst] = Cx<T>{T(0), T(0)};
+    }
   }
   const int lane = threadIdx.x % kWave;
   const int q = lane % L::P;
@@ -681,7 +751,7 @@ __device__ __forceinline__ void z_forward_body(const FastArgs<T>& a, int bid) {
       tile[s * LZ + phys<T>(zz0 + ZS * k)] = s < ns ? w[k] : Cx<T>{T(0), T(0)};
     }
   }
-  if (threadIdx.x < NZ) twl[threadIdx.x] = a.twZ[threadIdx.x];
+  for (int t = threadIdx.x; t < NZ; t += NT) twl[t] = a.twZ[t];
   if (a.hasRuns) {
     // sticks stored as runs: each lane writes its outputs straight to the
     // run (see z_backward_runs)
@@ -1461,11 +1531,14 @@ __device__ __forceinline__ void xsplit_backward_body(const FastArgs<T>& a, int b
   const int q = lane % P;
   const int line = S::line_of(wave, lane);
   // C2C: row y0 + line. R2C: rows y0 + 2 line (real part of the line's
-  // result) and y0 + 2 line + 1 (imaginary part); ny is even.
+  // result) and y0 + 2 line + 1 (imaginary part), which an odd ny leaves
+  // out of the last pair (its input is zero, its output not stored).
   const int y = line < S::LINES ? y0 + (RC ? 2 * line : line) : ny;  // spare lanes: no row
   const int yc = y < ny ? y : ny - 1 - (RC ? 1 : 0);
+  const bool hasB = yc + 1 < ny;
   const int ncol = a.ncol;
   const Cx<T>* yrow = a.ybuf + (static_cast<size_t>(z) * ny + yc) * a.pitchY;
+  const size_t offB = RC && hasB ? a.pitchY : 0;
   // compact row at its x positions; absent x read column 0 and are zeroed;
   // R2C: the upper half is the conjugate of x' = N - x, and the hermitian rows
   // a and b enter as a + i b (two real outputs from one complex transform)
@@ -1478,8 +1551,8 @@ __device__ __forceinline__ void xsplit_backward_body(const FastArgs<T>& a, int b
     Cx<T> w = yrow[uc];
     if (u >= ncol) w = Cx<T>{T(0), T(0)};
     if (RC) {
-      Cx<T> wb = yrow[a.pitchY + uc];
-      if (u >= ncol) wb = Cx<T>{T(0), T(0)};
+      Cx<T> wb = yrow[offB + uc];
+      if (u >= ncol || !hasB) wb = Cx<T>{T(0), T(0)};
       if (x == 0 || 2 * x == N) {
         w.im = T(0);
         wb.im = T(0);
@@ -1499,9 +1572,10 @@ __device__ __forceinline__ void xsplit_backward_body(const FastArgs<T>& a, int b
     const size_t row = (static_cast<size_t>(z) * ny + y) * N;
     if (RC) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        a.out[row + q + P * e] = v[e].re;
-        a.out[row + N + q + P * e] = v[e].im;
+      for (int e = 0; e < E; ++e) a.out[row + q + P * e] = v[e].re;
+      if (hasB) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) a.out[row + N + q + P * e] = v[e].im;
       }
     } else {
       Cx<T>* out = reinterpret_cast<Cx<T>*>(a.out) + row;
@@ -1527,14 +1601,20 @@ __device__ __forceinline__ void xsplit_forward_body(const FastArgs<T>& a, int bi
   const int q = lane % P;
   const int line = S::line_of(wave, lane);
   // C2C: row y0 + line. R2C: the real rows a = y0 + 2 line and b = a + 1
-  // enter as a + i b; ny is even.
+  // enter as a + i b; an odd ny leaves b out of the last pair (zero input,
+  // its spectrum not stored).
   const int y = line < S::LINES ? y0 + (RC ? 2 * line : line) : ny;  // spare lanes: no row
   const int yc = y < ny ? y : ny - 1 - (RC ? 1 : 0);
+  const bool hasB = yc + 1 < ny;
   const size_t row = (static_cast<size_t>(z) * ny + yc) * N;
   Cx<T> v[E];
   if (RC) {
+    const size_t rowB = hasB ? row + N : row;
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = Cx<T>{a.in[row + q + P * e], a.in[row + N + q + P * e]};
+    for (int e = 0; e < E; ++e) {
+      const T b = a.in[rowB + q + P * e];
+      v[e] = Cx<T>{a.in[row + q + P * e], hasB ? b : T(0)};
+    }
   } else {
     const Cx<T>* in = reinterpret_cast<const Cx<T>*>(a.in) + row;
 #pragma unroll
@@ -1565,7 +1645,7 @@ __device__ __forceinline__ void xsplit_forward_body(const FastArgs<T>& a, int bi
           const Cx<T> zp = lr[phys<T>(x == 0 ? 0 : N - x)];
           const T h = T(0.5);
           ya[uu[e]] = Cx<T>{h * (v[e].re + zp.re), h * (v[e].im - zp.im)};
-          ya[a.pitchY + uu[e]] = Cx<T>{h * (v[e].im + zp.im), h * (zp.re - v[e].re)};
+          if (hasB) ya[a.pitchY + uu[e]] = Cx<T>{h * (v[e].im + zp.im), h * (zp.re - v[e].re)};
         }
       }
     }
@@ -2176,44 +2256,16 @@ bool fast_length(int n) { return square_length(n) || split_only_length(n); }
     default: throw GPUFFTError();    \
   }
 
-#define SPFFT_FAST_SWITCH(N, CALL) \
-  switch (N) {                     \
-    case 32: CALL(32); break;      \
-    case 40: CALL(40); break;      \
-    case 48: CALL(48); break;      \
-    case 50: CALL(50); break;      \
-    case 54: CALL(54); break;      \
-    case 56: CALL(56); break;      \
-    case 60: CALL(60); break;      \
-    case 64: CALL(64); break;      \
-    case 72: CALL(72); break;      \
-    case 80: CALL(80); break;      \
-    case 84: CALL(84); break;      \
-    case 90: CALL(90); break;      \
-    case 96: CALL(96); break;      \
-    case 98: CALL(98); break;      \
-    case 100: CALL(100); break;    \
-    case 108: CALL(108); break;    \
-    case 112: CALL(112); break;    \
-    case 120: CALL(120); break;    \
-    case 126: CALL(126); break;    \
-    case 128: CALL(128); break;    \
-    case 144: CALL(144); break;    \
-    case 150: CALL(150); break;    \
-    case 160: CALL(160); break;    \
-    case 162: CALL(162); break;    \
-    case 168: CALL(168); break;    \
-    case 180: CALL(180); break;    \
-    case 192: CALL(192); break;    \
-    case 196: CALL(196); break;    \
-    case 200: CALL(200); break;    \
-    case 216: CALL(216); break;    \
-    case 224: CALL(224); break;    \
-    case 240: CALL(240); break;    \
-    case 252: CALL(252); break;    \
-    case 256: CALL(256); break;    \
-    default: throw GPUFFTError();  \
-  }
+// The fixed lengths, in the two groups of the z kernels and the split plane
+// stage (group 0 holds the class-split lengths).
+#define SPFFT_LENGTHS_0(X)                                                              \
+  X(32) X(40) X(48) X(50) X(54) X(56) X(60) X(64) X(72) X(80) X(84) X(90) X(96) X(98)  \
+  X(100) X(108) X(112) X(120) X(126) X(128) X(144) X(150) X(160) X(162) X(168) X(180) \
+  X(192) X(196) X(200) X(216) X(224) X(240) X(252) X(256)
+#define SPFFT_LENGTHS_1(X)                                                               \
+  X(36) X(42) X(45) X(63) X(70) X(75) X(81) X(105) X(125) X(135) X(140) X(147) X(175)   \
+  X(189) X(210) X(225) X(243) X(270) X(280) X(288) X(300) X(320) X(336) X(360) X(384)  \
+  X(400) X(420) X(432) X(448) X(450) X(480) X(500) X(512)
 
 }  // namespace
 
@@ -2265,14 +2317,38 @@ void z_batch_n(bool forward, bool rc, const FastArgs<T>* batch, const int2* map,
   check_launch();
 }
 
+// One call into the z kernels of length NZ.
+template <typename T, int NZ>
+void z_call(const ZCall<T>& c) {
+  switch (c.op) {
+    case ZCall<T>::Backward: z_backward_n<T, NZ>(*c.args, c.stream); break;
+    case ZCall<T>::Forward: z_forward_n<T, NZ>(*c.args, c.stream); break;
+    case ZCall<T>::Batch:
+      z_batch_n<T, NZ>(c.forward, c.rc, c.args, c.map, c.blocks, c.useRuns, c.stream);
+      break;
+    case ZCall<T>::Geometry:
+      c.geom[0] = ZShape<T, NZ>::S;
+      c.geom[1] = ZShape<T, NZ>::LZ;
+      c.geom[2] = pad_shift<T>();
+      break;
+  }
+}
+
+template <typename T>
+void z_dispatch(int nz, const ZCall<T>& c) {
+  if (!z_group0(nz, c) && !z_group1(nz, c)) throw GPUFFTError();
+}
+
 template <typename T>
 void fast_z_geometry(int nz, int* sticksPerBlock, int* lineLength, int* padShift) {
-#define SPFFT_GEOM(N)                          \
-  *sticksPerBlock = ZShape<T, N>::S;           \
-  *lineLength = ZShape<T, N>::LZ;              \
-  *padShift = pad_shift<T>()
-  SPFFT_FAST_SWITCH(nz, SPFFT_GEOM)
-#undef SPFFT_GEOM
+  int g[3] = {};
+  ZCall<T> c;
+  c.op = ZCall<T>::Geometry;
+  c.geom = g;
+  z_dispatch(nz, c);
+  *sticksPerBlock = g[0];
+  *lineLength = g[1];
+  *padShift = g[2];
 }
 
 template <typename T>
@@ -2294,16 +2370,20 @@ bool class_split_supported(int nx, int ny, int ncol, bool r2c) {
 
 template <typename T>
 void launch_fast_z_backward(int nz, const FastArgs<T>& a, hipStream_t stream) {
-#define SPFFT_CALL(N) z_backward_n<T, N>(a, stream)
-  SPFFT_FAST_SWITCH(nz, SPFFT_CALL)
-#undef SPFFT_CALL
+  ZCall<T> c;
+  c.op = ZCall<T>::Backward;
+  c.args = &a;
+  c.stream = stream;
+  z_dispatch(nz, c);
 }
 
 template <typename T>
 void launch_fast_z_forward(int nz, const FastArgs<T>& a, hipStream_t stream) {
-#define SPFFT_CALL(N) z_forward_n<T, N>(a, stream)
-  SPFFT_FAST_SWITCH(nz, SPFFT_CALL)
-#undef SPFFT_CALL
+  ZCall<T> c;
+  c.op = ZCall<T>::Forward;
+  c.args = &a;
+  c.stream = stream;
+  z_dispatch(nz, c);
 }
 
 template <typename T>
@@ -2334,9 +2414,16 @@ int fast_xy_pitch_a(int n, int ncol) {
 template <typename T>
 void launch_fast_z_batch(bool forward, bool r2c, int nz, const FastArgs<T>* batch,
                          const int2* map, int blocks, bool useRuns, hipStream_t stream) {
-#define SPFFT_CALL(N) z_batch_n<T, N>(forward, r2c, batch, map, blocks, useRuns ? 1 : 0, stream)
-  SPFFT_FAST_SWITCH(nz, SPFFT_CALL)
-#undef SPFFT_CALL
+  ZCall<T> c;
+  c.op = ZCall<T>::Batch;
+  c.args = batch;
+  c.map = map;
+  c.blocks = blocks;
+  c.useRuns = useRuns ? 1 : 0;
+  c.forward = forward;
+  c.rc = r2c;
+  c.stream = stream;
+  z_dispatch(nz, c);
 }
 
 template <typename T>
@@ -2540,19 +2627,38 @@ int split_pitch(int ncol, int nx) {
   return p <= nx ? p : nx;
 }
 
+// One call into the split plane stage of length N (the y kernel of columns
+// of length N, or the x kernel of rows of length N).
+template <typename T, int N>
+void split_call(const SplitCall<T>& c) {
+  if (!c.batch) {
+    if (c.y) {
+      ysplit_n<T, N>(c.forward, *c.args, c.stream);
+    } else {
+      xsplit_n<T, N>(c.forward, *c.args, c.stream);
+    }
+  } else if (c.y) {
+    ysplit_batch_n<T, N>(c.forward, c.rc, c.args, c.ntrans, c.nz, c.maxNcol, c.stream);
+  } else {
+    xsplit_batch_n<T, N>(c.forward, c.rc, c.args, c.ntrans, c.ny, c.nz, c.stream);
+  }
+}
+
+template <typename T>
+void split_dispatch(int n, const SplitCall<T>& c) {
+  if (!split_group0(n, c) && !split_group1(n, c)) throw GPUFFTError();
+}
+
 template <typename T>
 void launch_split_xy(bool forward, int nx, int ny, const FastArgs<T>& a, hipStream_t stream) {
   // backward: y then x; forward: x then y
   for (int k = 0; k < 2; ++k) {
-    if ((k == 0) != forward) {
-#define SPFFT_CALL(N) ysplit_n<T, N>(forward, a, stream)
-      SPFFT_FAST_SWITCH(ny, SPFFT_CALL)
-#undef SPFFT_CALL
-    } else {
-#define SPFFT_CALL(N) xsplit_n<T, N>(forward, a, stream)
-      SPFFT_FAST_SWITCH(nx, SPFFT_CALL)
-#undef SPFFT_CALL
-    }
+    SplitCall<T> c;
+    c.y = (k == 0) != forward;
+    c.forward = forward;
+    c.args = &a;
+    c.stream = stream;
+    split_dispatch(c.y ? ny : nx, c);
   }
 }
 
@@ -2560,17 +2666,38 @@ template <typename T>
 void launch_split_xy_batch(bool forward, bool r2c, int nx, int ny, const FastArgs<T>* batch,
                            int ntrans, int nz, int maxNcol, hipStream_t stream) {
   for (int k = 0; k < 2; ++k) {
-    if ((k == 0) != forward) {
-#define SPFFT_CALL(N) ysplit_batch_n<T, N>(forward, r2c, batch, ntrans, nz, maxNcol, stream)
-      SPFFT_FAST_SWITCH(ny, SPFFT_CALL)
-#undef SPFFT_CALL
-    } else {
-#define SPFFT_CALL(N) xsplit_batch_n<T, N>(forward, r2c, batch, ntrans, ny, nz, stream)
-      SPFFT_FAST_SWITCH(nx, SPFFT_CALL)
-#undef SPFFT_CALL
-    }
+    SplitCall<T> c;
+    c.y = (k == 0) != forward;
+    c.forward = forward;
+    c.batch = true;
+    c.rc = r2c;
+    c.args = batch;
+    c.ntrans = ntrans;
+    c.ny = ny;
+    c.nz = nz;
+    c.maxNcol = maxNcol;
+    c.stream = stream;
+    split_dispatch(c.y ? ny : nx, c);
   }
 }
+
+// Dispatchers of one group of lengths.
+#define SPFFT_Z_CASE(N) \
+  case N:               \
+    z_call<T, N>(c);    \
+    return true;
+#define SPFFT_SPLIT_CASE(N) \
+  case N:                   \
+    split_call<T, N>(c);    \
+    return true;
+#define SPFFT_DEFINE_GROUP(NAME, CALL, TYPE, CASE, LENGTHS) \
+  bool NAME(int n, const CALL<TYPE>& c) {                   \
+    using T = TYPE;                                         \
+    switch (n) {                                            \
+      LENGTHS(CASE)                                         \
+      default: return false;                                \
+    }                                                       \
+  }
 
 // z kernels
 #define SPFFT_INSTANTIATE_FAST_Z(T)                                                      \
@@ -2600,24 +2727,37 @@ void launch_split_xy_batch(bool forward, bool r2c, int nx, int ny, const FastArg
   template void launch_fast_xy_backward<T>(int, int, const FastArgs<T>&, hipStream_t);   \
   template void launch_fast_xy_forward<T>(int, int, const FastArgs<T>&, hipStream_t);
 
-// One precision and one kernel family per translation unit, so that they
-// build in parallel: the fast_kernels_*.hip files include this file with
-// SPFFT_FAST_UNIT set (1 to 5); compiled alone it is unit 0.
+// One precision and one kernel family (and, for the z and split kernels, one
+// group of lengths) per translation unit, so that they build in parallel:
+// the fast_kernels_*.hip files include this file with SPFFT_FAST_UNIT set (1
+// to 9); compiled alone it is unit 0.
 #ifndef SPFFT_FAST_UNIT
 #define SPFFT_FAST_UNIT 0
 #endif
 #if SPFFT_FAST_UNIT == 0
 SPFFT_INSTANTIATE_FAST_PLANE(double)
 #elif SPFFT_FAST_UNIT == 1
+SPFFT_DEFINE_GROUP(z_group0, ZCall, double, SPFFT_Z_CASE, SPFFT_LENGTHS_0)
 SPFFT_INSTANTIATE_FAST_Z(double)
 #elif SPFFT_FAST_UNIT == 2
+SPFFT_DEFINE_GROUP(split_group0, SplitCall, double, SPFFT_SPLIT_CASE, SPFFT_LENGTHS_0)
 SPFFT_INSTANTIATE_FAST_SPLIT(double)
 #elif SPFFT_FAST_UNIT == 3
 SPFFT_INSTANTIATE_FAST_PLANE(float)
 #elif SPFFT_FAST_UNIT == 4
+SPFFT_DEFINE_GROUP(z_group0, ZCall, float, SPFFT_Z_CASE, SPFFT_LENGTHS_0)
 SPFFT_INSTANTIATE_FAST_Z(float)
-#else
+#elif SPFFT_FAST_UNIT == 5
+SPFFT_DEFINE_GROUP(split_group0, SplitCall, float, SPFFT_SPLIT_CASE, SPFFT_LENGTHS_0)
 SPFFT_INSTANTIATE_FAST_SPLIT(float)
+#elif SPFFT_FAST_UNIT == 6
+SPFFT_DEFINE_GROUP(z_group1, ZCall, double, SPFFT_Z_CASE, SPFFT_LENGTHS_1)
+#elif SPFFT_FAST_UNIT == 7
+SPFFT_DEFINE_GROUP(split_group1, SplitCall, double, SPFFT_SPLIT_CASE, SPFFT_LENGTHS_1)
+#elif SPFFT_FAST_UNIT == 8
+SPFFT_DEFINE_GROUP(z_group1, ZCall, float, SPFFT_Z_CASE, SPFFT_LENGTHS_1)
+#else
+SPFFT_DEFINE_GROUP(split_group1, SplitCall, float, SPFFT_SPLIT_CASE, SPFFT_LENGTHS_1)
 #endif
 
 }  // namespace spfft

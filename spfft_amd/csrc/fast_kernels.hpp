@@ -97,9 +97,11 @@ inline int fast_pitch(int nst) {
 }
 
 // True when (nx, ny, nz) have fast kernels: z kernels and the split plane
-// stage for every length in {32, 40, 48, 50, 54, 56, 60, 64, 72, 80, 84, 90,
-// 96, 98, 100, 108, 112, 120, 126, 128, 144, 150, 160, 162, 168, 180, 192,
-// 196, 200, 216, 224, 240, 252, 256}.
+// stage for every length in {32, 36, 40, 42, 45, 48, 50, 54, 56, 60, 63, 64,
+// 70, 72, 75, 80, 81, 84, 90, 96, 98, 100, 105, 108, 112, 120, 125, 126, 128,
+// 135, 140, 144, 147, 150, 160, 162, 168, 175, 180, 189, 192, 196, 200, 210,
+// 216, 224, 225, 240, 243, 252, 256, 270, 280, 288, 300, 320, 336, 360, 384,
+// 400, 420, 432, 448, 450, 480, 500, 512}.
 template <typename T>
 bool fast_path_supported(int nx, int ny, int nz, int ncol, bool r2c);
 // True when the class-split plane kernels serve the plane too: nx = ny in

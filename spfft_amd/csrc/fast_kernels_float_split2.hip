@@ -1,0 +1,5 @@
+// Single-precision split plane stage of the fixed-length path, second group of
+// lengths (36 to 512): the source of fast_kernels.hip compiled as a
+// translation unit of its own, so that the units build in parallel.
+#define SPFFT_FAST_UNIT 9
+#include "fast_kernels.hip"
