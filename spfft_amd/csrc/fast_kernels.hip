@@ -27,7 +27,7 @@ namespace spfft {
 extern __shared__ __attribute__((aligned(16))) unsigned char spfft_lds_fast[];
 
 // One call into the z kernels or the split plane stage of one length. Those
-// kernels are compiled in two groups of lengths (two translation units per
+// kernels are compiled in three groups of lengths (three translation units per
 // precision and family, so that they build in parallel); each group's
 // dispatcher launches the call and returns true when it holds the length.
 template <typename T>
@@ -60,6 +60,10 @@ bool split_group0(int n, const SplitCall<double>& c);
 bool split_group0(int n, const SplitCall<float>& c);
 bool split_group1(int n, const SplitCall<double>& c);
 bool split_group1(int n, const SplitCall<float>& c);
+bool z_group2(int n, const ZCall<double>& c);
+bool z_group2(int n, const ZCall<float>& c);
+bool split_group2(int n, const SplitCall<double>& c);
+bool split_group2(int n, const SplitCall<float>& c);
 
 namespace {
 
@@ -91,14 +95,19 @@ struct Line {
 };
 
 constexpr int wave_radix(int m, int e) {
-  return (m % 8 == 0 && e % 8 == 0)   ? 8
-         : (m % 4 == 0 && e % 4 == 0) ? 4
-         : (m % 2 == 0 && e % 2 == 0) ? 2
-         : (m % 3 == 0 && e % 3 == 0) ? 3
-         : (m % 5 == 0 && e % 5 == 0) ? 5
-         : (m % 7 == 0 && e % 7 == 0) ? 7
-                                      : -1;
+  return (m % 8 == 0 && e % 8 == 0)     ? 8
+         : (m % 4 == 0 && e % 4 == 0)   ? 4
+         : (m % 2 == 0 && e % 2 == 0)   ? 2
+         : (m % 3 == 0 && e % 3 == 0)   ? 3
+         : (m % 5 == 0 && e % 5 == 0)   ? 5
+         : (m % 7 == 0 && e % 7 == 0)   ? 7
+         : (m % 11 == 0 && e % 11 == 0) ? 11
+         : (m % 13 == 0 && e % 13 == 0) ? 13
+                                        : -1;
 }
+// Radices without a constant butterfly: their DFT reads W_R from the line's
+// own twiddle table (W_N).
+constexpr bool table_radix(int r) { return r == 7 || r == 11 || r == 13; }
 // True when every pass of the wave-level schedule of an n-point line with e
 // elements per lane finds a radix dividing both the remaining length and e,
 // and the n / e lanes of a line fit a wavefront.
@@ -127,13 +136,15 @@ struct MixedSchedule {
   bool ok = false;
 };
 constexpr int dividing_pick(int m, int e) {
-  return (m % 7 == 0 && e % 7 == 0)   ? 7
-         : (m % 5 == 0 && e % 5 == 0) ? 5
-         : (m % 3 == 0 && e % 3 == 0) ? 3
-         : (m % 8 == 0 && e % 8 == 0) ? 8
-         : (m % 4 == 0 && e % 4 == 0) ? 4
-         : (m % 2 == 0 && e % 2 == 0) ? 2
-                                      : -1;
+  return (m % 13 == 0 && e % 13 == 0)   ? 13
+         : (m % 11 == 0 && e % 11 == 0) ? 11
+         : (m % 7 == 0 && e % 7 == 0)   ? 7
+         : (m % 5 == 0 && e % 5 == 0)   ? 5
+         : (m % 3 == 0 && e % 3 == 0)   ? 3
+         : (m % 8 == 0 && e % 8 == 0)   ? 8
+         : (m % 4 == 0 && e % 4 == 0)   ? 4
+         : (m % 2 == 0 && e % 2 == 0)   ? 2
+                                        : -1;
 }
 constexpr MixedSchedule mixed_schedule(int n, int e) {
   MixedSchedule s;
@@ -145,10 +156,10 @@ constexpr MixedSchedule mixed_schedule(int n, int e) {
   if (last < 0) return s;
   s.radix[s.count++] = first;
   int m = n / (first * last);
-  const int cand[6] = {8, 4, 2, 3, 5, 7};
+  const int cand[8] = {8, 4, 2, 3, 5, 7, 11, 13};
   while (m > 1) {
     int best = -1, bestCost = 1 << 30;
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < 8; ++i) {
       const int r = cand[i];
       if (m % r != 0) continue;
       const int cost = r * ((n / r + p - 1) / p);
@@ -191,7 +202,9 @@ constexpr bool schedule_ok(int n, int e) {
 // the line's twiddle table (56 = 4 x 14: 7 2 2 2, 84 = 4 x 21: 7 2 2 3, ...).
 // Odd lengths (45, 63, 75, 81, 105, 125, 135, 147, 175, 189, 225, 243) run
 // on the same lines; an R2C plane with an odd row count leaves the last
-// packed row pair with one row.
+// packed row pair with one row. Lengths with a factor 11 or 13 (44 to 462)
+// have lines of 22 or 26 elements per lane and one radix-11 or radix-13
+// pass (44 = 2 x 22: 2 2 11, 78 = 3 x 26: 13 3 2, ...).
 struct LaneElems {
   int full64, full32, z64, z32;
 };
@@ -200,62 +213,88 @@ constexpr LaneElems lane_elems(int n) {
     case 36: return {18, 18, 18, 18};
     case 40: return {10, 10, 10, 10};
     case 42: return {21, 21, 21, 21};
+    case 44: return {22, 22, 22, 22};
     case 45: return {15, 15, 15, 15};
     case 48: return {12, 12, 12, 12};
     case 50: return {10, 10, 10, 10};
+    case 52: return {26, 26, 26, 26};
     case 54: return {18, 18, 18, 18};
     case 56: return {14, 14, 14, 14};
     case 60: return {15, 15, 15, 15};
     case 63: return {21, 21, 21, 21};
+    case 66: return {22, 22, 22, 22};
     case 70: return {14, 14, 14, 14};
     case 72: return {18, 18, 18, 18};
     case 75: return {15, 15, 15, 15};
+    case 78: return {26, 26, 26, 26};
     case 80: return {10, 20, 10, 10};
     case 81: return {27, 27, 27, 27};
     case 84: return {21, 21, 21, 21};
+    case 88: return {22, 22, 22, 22};
     case 90: return {15, 15, 15, 15};
     case 98: return {14, 14, 14, 14};
     case 100: return {25, 25, 25, 25};
+    case 104: return {26, 26, 26, 26};
     case 105: return {15, 15, 15, 15};
     case 108: return {27, 27, 27, 27};
+    case 110: return {22, 22, 22, 22};
     case 112: return {14, 14, 14, 14};
     case 120: return {15, 15, 15, 15};
     case 125: return {25, 25, 25, 25};
     case 126: return {21, 21, 21, 21};
+    case 130: return {26, 26, 26, 26};
+    case 132: return {22, 22, 22, 22};
     case 135: return {15, 15, 15, 15};
     case 140: return {14, 14, 14, 14};
     case 144: return {18, 18, 18, 18};
     case 147: return {21, 21, 21, 21};
     case 150: return {15, 15, 15, 15};
+    case 154: return {22, 22, 22, 22};
+    case 156: return {26, 26, 26, 26};
     case 160: return {10, 20, 10, 10};
     case 162: return {27, 27, 27, 27};
     case 168: return {21, 21, 21, 21};
     case 175: return {25, 25, 25, 25};
+    case 176: return {22, 22, 22, 22};
     case 180: return {15, 15, 15, 15};
+    case 182: return {26, 26, 26, 26};
     case 189: return {21, 21, 21, 21};
     case 196: return {28, 28, 28, 28};
+    case 198: return {22, 22, 22, 22};
     case 200: return {25, 25, 25, 25};
+    case 208: return {26, 26, 26, 26};
     case 210: return {21, 21, 21, 21};
     case 216: return {27, 27, 27, 27};
+    case 220: return {22, 22, 22, 22};
     case 224: return {14, 14, 14, 14};
     case 225: return {15, 15, 15, 15};
+    case 234: return {26, 26, 26, 26};
     case 240: return {15, 15, 15, 15};
     case 243: return {27, 27, 27, 27};
     case 252: return {21, 21, 21, 21};
     case 256: return {8, 8, 4, 8};
+    case 260: return {26, 26, 26, 26};
+    case 264: return {22, 22, 22, 22};
     case 270: return {27, 27, 27, 27};
     case 280: return {14, 14, 14, 14};
     case 288: return {18, 18, 18, 18};
     case 300: return {15, 15, 15, 15};
+    case 312: return {26, 26, 26, 26};
     case 320: return {10, 20, 10, 10};
+    case 330: return {22, 22, 22, 22};
     case 336: return {21, 21, 21, 21};
+    case 352: return {22, 22, 22, 22};
     case 360: return {18, 18, 18, 18};
     case 384: return {12, 24, 12, 12};
+    case 390: return {26, 26, 26, 26};
     case 400: return {25, 25, 25, 25};
+    case 416: return {26, 26, 26, 26};
     case 420: return {21, 21, 21, 21};
     case 432: return {27, 27, 27, 27};
+    case 440: return {22, 22, 22, 22};
     case 448: return {14, 14, 14, 14};
     case 450: return {15, 15, 15, 15};
+    case 462: return {22, 22, 22, 22};
     case 480: return {15, 15, 15, 15};
     case 500: return {25, 25, 25, 25};
     case 512: return {8, 8, 8, 8};
@@ -387,7 +426,7 @@ __device__ __forceinline__ void wave_fft_mixed(Cx<T>* v, Cx<T>* line, int es, in
   static_assert(S.ok, "no mixed schedule");
   constexpr int PL = N / E;
   constexpr int R = S.radix[PASS];
-  static_assert(R != 7 || TWS == 1, "table butterflies need the line's own table");
+  static_assert(!table_radix(R) || TWS == 1, "table butterflies need the line's own table");
   constexpr int NB = N / R;  // butterflies of the pass
   constexpr bool FIRST = PASS == 0, LAST = PASS == S.count - 1;
   static_assert(!(FIRST || LAST) || E % R == 0, "first and last radices divide E");
@@ -447,8 +486,8 @@ __device__ __forceinline__ void wave_fft(Cx<T>* v, Cx<T>* line, int es, int q,
   constexpr int PL = N / E;
   constexpr int R = PE > 0 ? full_radix(N / NS, E) : wave_radix(N / NS, E);
   static_assert(R > 0, "no radix divides the lane's element count");
-  // radices without a constant butterfly (7) read W_R from the line's table
-  static_assert(R != 7 || TWS == 1, "table butterflies need the line's own table");
+  // radices without a constant butterfly (7, 11, 13) read W_R from the line's table
+  static_assert(!table_radix(R) || TWS == 1, "table butterflies need the line's own table");
   constexpr int EB = E / R;
   constexpr bool LAST = NS * R == N;
   constexpr int STEP = N / (NS * R) * TWS;
@@ -2256,8 +2295,9 @@ bool fast_length(int n) { return square_length(n) || split_only_length(n); }
     default: throw GPUFFTError();    \
   }
 
-// The fixed lengths, in the two groups of the z kernels and the split plane
-// stage (group 0 holds the class-split lengths).
+// The fixed lengths, in the three groups of the z kernels and the split plane
+// stage (group 0 holds the class-split lengths; group 2 the lengths with a
+// factor 11 or 13).
 #define SPFFT_LENGTHS_0(X)                                                              \
   X(32) X(40) X(48) X(50) X(54) X(56) X(60) X(64) X(72) X(80) X(84) X(90) X(96) X(98)  \
   X(100) X(108) X(112) X(120) X(126) X(128) X(144) X(150) X(160) X(162) X(168) X(180) \
@@ -2266,6 +2306,10 @@ bool fast_length(int n) { return square_length(n) || split_only_length(n); }
   X(36) X(42) X(45) X(63) X(70) X(75) X(81) X(105) X(125) X(135) X(140) X(147) X(175)   \
   X(189) X(210) X(225) X(243) X(270) X(280) X(288) X(300) X(320) X(336) X(360) X(384)  \
   X(400) X(420) X(432) X(448) X(450) X(480) X(500) X(512)
+#define SPFFT_LENGTHS_2(X)                                                                \
+  X(44) X(52) X(66) X(78) X(88) X(104) X(110) X(130) X(132) X(154) X(156) X(176) X(182)  \
+  X(198) X(208) X(220) X(234) X(260) X(264) X(312) X(330) X(352) X(390) X(416) X(440)   \
+  X(462)
 
 }  // namespace
 
@@ -2336,7 +2380,7 @@ void z_call(const ZCall<T>& c) {
 
 template <typename T>
 void z_dispatch(int nz, const ZCall<T>& c) {
-  if (!z_group0(nz, c) && !z_group1(nz, c)) throw GPUFFTError();
+  if (!z_group0(nz, c) && !z_group1(nz, c) && !z_group2(nz, c)) throw GPUFFTError();
 }
 
 template <typename T>
@@ -2646,7 +2690,7 @@ void split_call(const SplitCall<T>& c) {
 
 template <typename T>
 void split_dispatch(int n, const SplitCall<T>& c) {
-  if (!split_group0(n, c) && !split_group1(n, c)) throw GPUFFTError();
+  if (!split_group0(n, c) && !split_group1(n, c) && !split_group2(n, c)) throw GPUFFTError();
 }
 
 template <typename T>
@@ -2730,7 +2774,7 @@ void launch_split_xy_batch(bool forward, bool r2c, int nx, int ny, const FastArg
 // One precision and one kernel family (and, for the z and split kernels, one
 // group of lengths) per translation unit, so that they build in parallel:
 // the fast_kernels_*.hip files include this file with SPFFT_FAST_UNIT set (1
-// to 9); compiled alone it is unit 0.
+// to 13); compiled alone it is unit 0.
 #ifndef SPFFT_FAST_UNIT
 #define SPFFT_FAST_UNIT 0
 #endif
@@ -2756,8 +2800,16 @@ SPFFT_DEFINE_GROUP(z_group1, ZCall, double, SPFFT_Z_CASE, SPFFT_LENGTHS_1)
 SPFFT_DEFINE_GROUP(split_group1, SplitCall, double, SPFFT_SPLIT_CASE, SPFFT_LENGTHS_1)
 #elif SPFFT_FAST_UNIT == 8
 SPFFT_DEFINE_GROUP(z_group1, ZCall, float, SPFFT_Z_CASE, SPFFT_LENGTHS_1)
-#else
+#elif SPFFT_FAST_UNIT == 9
 SPFFT_DEFINE_GROUP(split_group1, SplitCall, float, SPFFT_SPLIT_CASE, SPFFT_LENGTHS_1)
+#elif SPFFT_FAST_UNIT == 10
+SPFFT_DEFINE_GROUP(z_group2, ZCall, double, SPFFT_Z_CASE, SPFFT_LENGTHS_2)
+#elif SPFFT_FAST_UNIT == 11
+SPFFT_DEFINE_GROUP(split_group2, SplitCall, double, SPFFT_SPLIT_CASE, SPFFT_LENGTHS_2)
+#elif SPFFT_FAST_UNIT == 12
+SPFFT_DEFINE_GROUP(z_group2, ZCall, float, SPFFT_Z_CASE, SPFFT_LENGTHS_2)
+#else
+SPFFT_DEFINE_GROUP(split_group2, SplitCall, float, SPFFT_SPLIT_CASE, SPFFT_LENGTHS_2)
 #endif
 
 }  // namespace spfft

@@ -95,21 +95,91 @@ __device__ __forceinline__ Cx<T> kscale(Cx<T> a, double k) {
 
 // ---- butterflies: in-register DFT of length R with sign -1 (fwd) / +1 (bwd)
 
+// acc + a c, c the real (IM = false) or imaginary (IM = true) part of a table
+// entry, negated when NEG; a compensated FP32 entry adds its lo part first.
+template <bool IM, bool NEG, typename T>
+__device__ __forceinline__ Cx<T> cfma_part(Cx<T> acc, Cx<T> a, Cx<T> t) {
+  const T c = NEG ? -(IM ? t.im : t.re) : (IM ? t.im : t.re);
+  return {fma(a.re, c, acc.re), fma(a.im, c, acc.im)};
+}
+template <bool IM, bool NEG, typename T>
+__device__ __forceinline__ Cx<T> cfma_part(Cx<T> acc, Cx<T> a, TwPair<T> t) {
+  const T hi = NEG ? -(IM ? t.hi.im : t.hi.re) : (IM ? t.hi.im : t.hi.re);
+  const T lo = NEG ? -(IM ? t.lo.im : t.lo.re) : (IM ? t.lo.im : t.lo.re);
+  return {fma(a.re, hi, fma(a.re, lo, acc.re)), fma(a.im, hi, fma(a.im, lo, acc.im))};
+}
+
+// Term r of output q of the odd-radix butterfly below (compile-time indices).
+template <typename T, int R, int Q, int I>
+struct OddTerms {
+  static constexpr int H = (R - 1) / 2;
+  static constexpr int r = I + 1;
+  static constexpr int m = (r * Q) % R;  // W_R^m
+  static constexpr bool UP = m > H;      // W_R^m = conj W_R^(R - m)
+  static constexpr int mm = UP ? R - m : m;
+  __device__ __forceinline__ static void run(Cx<T>& x, Cx<T>& y, const Cx<T>* a, const Cx<T>* b,
+                                             const Tw<T>* t) {
+    if constexpr (m == 0) {  // composite R: W_R^0 = 1
+      x = cadd(x, a[r]);
+    } else {
+      x = cfma_part<false, false>(x, a[r], t[mm]);
+      y = cfma_part<true, UP>(y, b[r], t[mm]);
+    }
+    if constexpr (I + 1 < H) OddTerms<T, R, Q, I + 1>::run(x, y, a, b, t);
+  }
+};
+template <typename T, int R, int Q>
+struct OddOutputs {
+  __device__ __forceinline__ static void run(Cx<T>* v, const Cx<T>* a, const Cx<T>* b,
+                                             const Tw<T>* t, Cx<T> v0) {
+    Cx<T> x = v0, y = {T(0), T(0)};
+    OddTerms<T, R, Q, 0>::run(x, y, a, b, t);
+    v[Q] = Cx<T>{x.re - y.im, x.im + y.re};          // x + i y
+    v[R - Q] = Cx<T>{x.re + y.im, x.im - y.re};      // x - i y
+    if constexpr (Q + 1 <= (R - 1) / 2) OddOutputs<T, R, Q + 1>::run(v, a, b, t, v0);
+  }
+};
+
 template <typename T, int R, bool BWD>
 struct Butterfly {
-  // Direct DFT from the length-n table: W_R^m = W_n^(m n / R).
+  // DFT of length R from the length-n table: W_R^m = W_n^(m n / R).
   __device__ __forceinline__ static void run(Cx<T>* v, const Tw<T>* __restrict__ w, int n) {
-    Cx<T> out[R];
     const int step = n / R;
+    if constexpr (R % 2 == 1) {
+      // Odd R: W_R^(R - m) = conj W_R^m, so with a_r = v_r + v_(R-r) and
+      // b_r = v_r - v_(R-r) (r = 1 .. H = (R - 1) / 2), output q and R - q
+      // are x_q +- i y_q, x_q = v_0 + sum_r a_r Re W^(rq), y_q = sum_r b_r
+      // Im W^(rq): H roots and 4 H^2 real products instead of (R - 1)^2
+      // complex ones.
+      constexpr int H = (R - 1) / 2;
+      Cx<T> a[H + 1], b[H + 1];
+      Tw<T> t[H + 1];
+      const Cx<T> v0 = v[0];
+      Cx<T> s = v0;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      Cx<T> acc = v[0];
+      for (int r = 1; r <= H; ++r) {
+        a[r] = cadd(v[r], v[R - r]);
+        b[r] = csub(v[r], v[R - r]);
+        t[r] = twiddle<BWD>(w, r * step);
+        s = cadd(s, a[r]);
+      }
+      OddOutputs<T, R, 1>::run(v, a, b, t, v0);
+      v[0] = s;
+    } else {
+      // direct DFT
+      Cx<T> out[R];
 #pragma unroll
-      for (int r = 1; r < R; ++r) acc = cadd(acc, cmul(v[r], twiddle<BWD>(w, ((r * q) % R) * step)));
-      out[q] = acc;
+      for (int q = 0; q < R; ++q) {
+        Cx<T> acc = v[0];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          acc = cadd(acc, cmul(v[r], twiddle<BWD>(w, ((r * q) % R) * step)));
+        }
+        out[q] = acc;
+      }
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[q] = out[q];
     }
-#pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = out[q];
   }
 };
 
