@@ -279,14 +279,15 @@ def launch_floor(torch, kernel_us):
 
 def plain_lengths_bench(args, sp, torch, np):
     """Single C2C FP64 transforms at other plane-wave lengths, sphere radius
-    0.3125 N (40 at 128): 120, 100 and 144 (z kernels and the split plane
-    stage; 120 and 100 on mixed radix schedules) and 132 (a factor 11: generic
-    stage kernels). Backward+forward pairs/s and the fraction of the HBM roofline of
-    the pair's compulsory bytes, rotating over 8 transforms; `kernels` names
-    the kernels one pair launched."""
+    0.3125 N (40 at 128): 120, 100, 144 and 132 (z kernels and the split plane
+    stage; 120, 100 and 132 on mixed radix schedules, 132 with a radix-11
+    pass) and 143 (11 x 13: generic stage kernels). Backward+forward pairs/s
+    and the fraction of the HBM roofline of the pair's compulsory bytes,
+    rotating over 8 transforms; `kernels` names the kernels one pair
+    launched."""
     from tests.workloads import complex_values, sphere_triplets
     out = {}
-    for n in (120, 100, 144, 132):
+    for n in (120, 100, 144, 132, 143):
         dims = (n, n, n)
         t = sphere_triplets(0.3125 * n)
         trs, ins, outs = [], [], []
