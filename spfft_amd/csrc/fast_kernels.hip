@@ -194,12 +194,14 @@ constexpr bool schedule_ok(int n, int e) {
 // dividing schedules, every radix dividing the lane's elements (72, 144: 2 2
 // 2 (2) 3 3 on 18; 80, 160: 2 2 2 2 (2) 5 on 10, or 4 4 (2) 5 on 20; 48: 4 4
 // 3 on 12; 256: 8 8 4 on 8, z 4 4 4 4 on 4; 320: 2 2 2 2 2 2 5 on 10, or 4 4 4
-// 5 on 20; 384: 4 4 4 2 3 on 12, or 8 8 2 3 on 24; 512: 8 8 8 on 8; 36: 2 2 3
-// 3 on 18; 288: 2 2 2 2 2 3 3 on 18), or mixed schedules (mixed_schedule: 60
+// 5 on 20; 384: 4 4 4 2 3 on 12, or 8 8 2 3 on 24; 512: 8 8 8 on 8, 16 or 32;
+// 36: 2 2 3 3 on 18; 288: 2 2 2 2 2 3 3 on 18), or mixed schedules (mixed_schedule: 60
 // = 4 x 15 (5 4 3), 120 = 8 x 15 (5 8 3), 100 = 4 x 25 (5 2 2 5), ...).
 // Lanes per line not dividing the wave leave spare lanes (50 = 5 x 10, 90 = 6
 // x 15, 45 = 3 x 15, ...). Lengths with a factor 7 take radix-7 passes from
 // the line's twiddle table (56 = 4 x 14: 7 2 2 2, 84 = 4 x 21: 7 2 2 3, ...).
+// FP32 448 and 512 take longer lines (28; 16 and, for z, 32 elements) so
+// that a workgroup's row segments are 64 to 128 bytes (512^3: +36%).
 // Odd lengths (45, 63, 75, 81, 105, 125, 135, 147, 175, 189, 225, 243) run
 // on the same lines; an R2C plane with an odd row count leaves the last
 // packed row pair with one row. Lengths with a factor 11 or 13 (44 to 462)
@@ -292,12 +294,12 @@ constexpr LaneElems lane_elems(int n) {
     case 420: return {21, 21, 21, 21};
     case 432: return {27, 27, 27, 27};
     case 440: return {22, 22, 22, 22};
-    case 448: return {14, 14, 14, 14};
+    case 448: return {14, 28, 14, 28};
     case 450: return {15, 15, 15, 15};
     case 462: return {22, 22, 22, 22};
     case 480: return {15, 15, 15, 15};
     case 500: return {25, 25, 25, 25};
-    case 512: return {8, 8, 8, 8};
+    case 512: return {8, 16, 8, 32};
     default: return {0, 0, 0, 0};
   }
 }
