@@ -212,93 +212,93 @@ struct LaneElems {
 };
 constexpr LaneElems lane_elems(int n) {
   switch (n) {
-    case 36: return {18, 18, 18, 18};
+    case 36: return {9, 9, 9, 9};
     case 40: return {10, 10, 10, 10};
     case 42: return {21, 21, 21, 21};
     case 44: return {22, 22, 22, 22};
-    case 45: return {15, 15, 15, 15};
-    case 48: return {12, 12, 12, 12};
-    case 50: return {10, 10, 10, 10};
+    case 45: return {9, 9, 9, 9};
+    case 48: return {6, 6, 6, 6};
+    case 50: return {5, 5, 5, 5};
     case 52: return {26, 26, 26, 26};
-    case 54: return {18, 18, 18, 18};
+    case 54: return {6, 6, 6, 6};
     case 56: return {14, 14, 14, 14};
     case 60: return {15, 15, 15, 15};
-    case 63: return {21, 21, 21, 21};
+    case 63: return {9, 9, 9, 9};
     case 66: return {22, 22, 22, 22};
-    case 70: return {14, 14, 14, 14};
-    case 72: return {18, 18, 18, 18};
-    case 75: return {15, 15, 15, 15};
+    case 70: return {10, 10, 10, 10};
+    case 72: return {9, 9, 9, 9};
+    case 75: return {5, 5, 5, 5};
     case 78: return {26, 26, 26, 26};
     case 80: return {10, 20, 10, 10};
-    case 81: return {27, 27, 27, 27};
-    case 84: return {21, 21, 21, 21};
+    case 81: return {9, 9, 9, 9};
+    case 84: return {12, 12, 12, 12};
     case 88: return {22, 22, 22, 22};
-    case 90: return {15, 15, 15, 15};
-    case 98: return {14, 14, 14, 14};
-    case 100: return {25, 25, 25, 25};
+    case 90: return {10, 10, 10, 10};
+    case 98: return {7, 7, 7, 7};
+    case 100: return {5, 5, 5, 5};
     case 104: return {26, 26, 26, 26};
     case 105: return {15, 15, 15, 15};
-    case 108: return {27, 27, 27, 27};
+    case 108: return {9, 9, 9, 9};
     case 110: return {22, 22, 22, 22};
-    case 112: return {14, 14, 14, 14};
+    case 112: return {8, 8, 8, 8};
     case 120: return {15, 15, 15, 15};
-    case 125: return {25, 25, 25, 25};
-    case 126: return {21, 21, 21, 21};
+    case 125: return {5, 5, 5, 5};
+    case 126: return {6, 6, 6, 6};
     case 130: return {26, 26, 26, 26};
     case 132: return {22, 22, 22, 22};
-    case 135: return {15, 15, 15, 15};
+    case 135: return {9, 9, 9, 9};
     case 140: return {14, 14, 14, 14};
-    case 144: return {18, 18, 18, 18};
-    case 147: return {21, 21, 21, 21};
-    case 150: return {15, 15, 15, 15};
+    case 144: return {9, 9, 9, 9};
+    case 147: return {7, 7, 7, 7};
+    case 150: return {10, 10, 10, 10};
     case 154: return {22, 22, 22, 22};
     case 156: return {26, 26, 26, 26};
     case 160: return {10, 20, 10, 10};
-    case 162: return {27, 27, 27, 27};
+    case 162: return {9, 9, 9, 9};
     case 168: return {21, 21, 21, 21};
     case 175: return {25, 25, 25, 25};
     case 176: return {22, 22, 22, 22};
-    case 180: return {15, 15, 15, 15};
+    case 180: return {12, 15, 12, 15};
     case 182: return {26, 26, 26, 26};
-    case 189: return {21, 21, 21, 21};
-    case 196: return {28, 28, 28, 28};
+    case 189: return {9, 9, 9, 9};
+    case 196: return {14, 14, 14, 14};
     case 198: return {22, 22, 22, 22};
-    case 200: return {25, 25, 25, 25};
+    case 200: return {10, 10, 10, 10};
     case 208: return {26, 26, 26, 26};
-    case 210: return {21, 21, 21, 21};
-    case 216: return {27, 27, 27, 27};
+    case 210: return {10, 10, 10, 10};
+    case 216: return {12, 12, 12, 12};
     case 220: return {22, 22, 22, 22};
     case 224: return {14, 14, 14, 14};
     case 225: return {15, 15, 15, 15};
     case 234: return {26, 26, 26, 26};
     case 240: return {15, 15, 15, 15};
-    case 243: return {27, 27, 27, 27};
-    case 252: return {21, 21, 21, 21};
+    case 243: return {9, 9, 9, 9};
+    case 252: return {12, 12, 12, 12};
     case 256: return {8, 8, 4, 8};
     case 260: return {26, 26, 26, 26};
     case 264: return {22, 22, 22, 22};
-    case 270: return {27, 27, 27, 27};
+    case 270: return {9, 9, 9, 9};
     case 280: return {14, 14, 14, 14};
-    case 288: return {18, 18, 18, 18};
-    case 300: return {15, 15, 15, 15};
+    case 288: return {9, 18, 9, 18};
+    case 300: return {10, 10, 10, 10};
     case 312: return {26, 26, 26, 26};
     case 320: return {10, 20, 10, 10};
     case 330: return {22, 22, 22, 22};
     case 336: return {21, 21, 21, 21};
     case 352: return {22, 22, 22, 22};
-    case 360: return {18, 18, 18, 18};
+    case 360: return {12, 12, 12, 12};
     case 384: return {12, 24, 12, 12};
     case 390: return {26, 26, 26, 26};
     case 400: return {25, 25, 25, 25};
     case 416: return {26, 26, 26, 26};
-    case 420: return {21, 21, 21, 21};
+    case 420: return {15, 21, 15, 21};
     case 432: return {27, 27, 27, 27};
     case 440: return {22, 22, 22, 22};
     case 448: return {14, 28, 14, 28};
     case 450: return {15, 15, 15, 15};
     case 462: return {22, 22, 22, 22};
     case 480: return {15, 15, 15, 15};
-    case 500: return {25, 25, 25, 25};
+    case 500: return {10, 10, 10, 10};
     case 512: return {8, 16, 8, 32};
     default: return {0, 0, 0, 0};
   }
