@@ -190,29 +190,26 @@ constexpr bool schedule_ok(int n, int e) {
 
 // Lengths served by the split plane stage only (no class-split or whole-plane
 // kernels): elements per lane of their full-length lines and of their z lines
-// (FP64, FP32), chosen so that each line's schedule exists (schedule_ok):
-// dividing schedules, every radix dividing the lane's elements (72, 144: 2 2
-// 2 (2) 3 3 on 18; 80, 160: 2 2 2 2 (2) 5 on 10, or 4 4 (2) 5 on 20; 48: 4 4
-// 3 on 12; 256: 8 8 4 on 8, z 4 4 4 4 on 4; 320: 2 2 2 2 2 2 5 on 10, or 4 4 4
-// 5 on 20; 384: 4 4 4 2 3 on 12, or 8 8 2 3 on 24; 512: 8 8 8 on 8, 16 or 32;
-// 36: 2 2 3 3 on 18; 288: 2 2 2 2 2 3 3 on 18), or mixed schedules (mixed_schedule: 60
-// = 4 x 15 (5 4 3), 120 = 8 x 15 (5 8 3), 100 = 4 x 25 (5 2 2 5), ...).
-// Lanes per line not dividing the wave leave spare lanes (50 = 5 x 10, 90 = 6
-// x 15, 45 = 3 x 15, ...). Lengths with a factor 7 take radix-7 passes from
-// the line's twiddle table (56 = 4 x 14: 7 2 2 2, 84 = 4 x 21: 7 2 2 3, ...).
-// FP32 448 and 512 take longer lines (28; 16 and, for z, 32 elements) so
-// that a workgroup's row segments are 64 to 128 bytes (512^3: +36%).
-// Odd lengths (45, 63, 75, 81, 105, 125, 135, 147, 175, 189, 225, 243) run
-// on the same lines; an R2C plane with an odd row count leaves the last
-// packed row pair with one row. Lengths with a factor 11 or 13 (44 to 462)
-// have lines of 22 or 26 elements per lane and one radix-11 or radix-13
-// pass (44 = 2 x 22: 2 2 11, 78 = 3 x 26: 13 3 2, ...).
+// (FP64, FP32), chosen so that each line's schedule exists (schedule_ok), as
+// a dividing schedule (every radix divides the lane's elements, e.g. 80 = 8
+// lanes x 10: 2 2 2 2 5) or a mixed one (mixed_schedule, e.g. 120 = 8 x 15:
+// 5 8 3; 144 = 16 x 9: 3 2 2 2 2 3). Lanes per line not dividing the wave
+// leave spare lanes (50 = 10 x 5, 90 = 9 x 10, ...). Radix 7, 11 and 13 take
+// their roots from the line's twiddle table; lengths with a factor 11 or 13
+// need lines of 22 or 26 elements. Measured on the MI355X, lines of 5 to 14
+// elements run fastest (lines of 21 to 27 held up to 256 VGPRs: 81^3 FP64
+// 13.2k -> 23.5k pairs/s at 9 elements), except where that would take 20 or
+// more lanes per line with spare lanes (120, 240, 400, 432 in FP64; 288, 320
+// in FP32); FP32 448 and 512 take longer lines (28; 16 and, for z, 32) so
+// that a workgroup's row segments are 64 to 128 bytes (512^3: +36%). Odd
+// lengths run on the same lines; an R2C plane with an odd row count leaves
+// the last packed row pair with one row.
 struct LaneElems {
   int full64, full32, z64, z32;
 };
 constexpr LaneElems lane_elems(int n) {
   switch (n) {
-    case 36: return {9, 9, 9, 9};
+    case 36: return {6, 9, 6, 9};
     case 40: return {10, 10, 10, 10};
     case 42: return {21, 21, 21, 21};
     case 44: return {22, 22, 22, 22};
@@ -226,19 +223,19 @@ constexpr LaneElems lane_elems(int n) {
     case 63: return {9, 9, 9, 9};
     case 66: return {22, 22, 22, 22};
     case 70: return {10, 10, 10, 10};
-    case 72: return {9, 9, 9, 9};
+    case 72: return {6, 9, 6, 9};
     case 75: return {5, 5, 5, 5};
     case 78: return {26, 26, 26, 26};
-    case 80: return {10, 20, 10, 10};
+    case 80: return {10, 10, 10, 10};
     case 81: return {9, 9, 9, 9};
-    case 84: return {12, 12, 12, 12};
+    case 84: return {6, 12, 6, 12};
     case 88: return {22, 22, 22, 22};
     case 90: return {10, 10, 10, 10};
     case 98: return {7, 7, 7, 7};
     case 100: return {5, 5, 5, 5};
     case 104: return {26, 26, 26, 26};
     case 105: return {15, 15, 15, 15};
-    case 108: return {9, 9, 9, 9};
+    case 108: return {6, 9, 6, 9};
     case 110: return {22, 22, 22, 22};
     case 112: return {8, 8, 8, 8};
     case 120: return {15, 15, 15, 15};
@@ -250,12 +247,12 @@ constexpr LaneElems lane_elems(int n) {
     case 140: return {14, 14, 14, 14};
     case 144: return {9, 9, 9, 9};
     case 147: return {7, 7, 7, 7};
-    case 150: return {10, 10, 10, 10};
+    case 150: return {5, 10, 5, 10};
     case 154: return {22, 22, 22, 22};
     case 156: return {26, 26, 26, 26};
-    case 160: return {10, 20, 10, 10};
+    case 160: return {10, 10, 10, 10};
     case 162: return {9, 9, 9, 9};
-    case 168: return {21, 21, 21, 21};
+    case 168: return {21, 14, 21, 14};
     case 175: return {25, 25, 25, 25};
     case 176: return {22, 22, 22, 22};
     case 180: return {12, 15, 12, 15};
@@ -291,7 +288,7 @@ constexpr LaneElems lane_elems(int n) {
     case 390: return {26, 26, 26, 26};
     case 400: return {25, 25, 25, 25};
     case 416: return {26, 26, 26, 26};
-    case 420: return {15, 21, 15, 21};
+    case 420: return {15, 15, 15, 15};
     case 432: return {27, 27, 27, 27};
     case 440: return {22, 22, 22, 22};
     case 448: return {14, 28, 14, 28};
