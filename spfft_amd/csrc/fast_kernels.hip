@@ -209,50 +209,50 @@ struct LaneElems {
 };
 constexpr LaneElems lane_elems(int n) {
   switch (n) {
-    case 36: return {6, 9, 6, 9};
-    case 40: return {10, 10, 10, 10};
-    case 42: return {21, 21, 21, 21};
+    case 36: return {3, 3, 3, 3};
+    case 40: return {4, 4, 4, 4};
+    case 42: return {6, 6, 6, 6};
     case 44: return {22, 22, 22, 22};
-    case 45: return {9, 9, 9, 9};
-    case 48: return {6, 6, 6, 6};
+    case 45: return {3, 3, 3, 3};
+    case 48: return {4, 4, 4, 4};
     case 50: return {5, 5, 5, 5};
     case 52: return {26, 26, 26, 26};
-    case 54: return {6, 6, 6, 6};
-    case 56: return {14, 14, 14, 14};
-    case 60: return {15, 15, 15, 15};
-    case 63: return {9, 9, 9, 9};
+    case 54: return {3, 3, 3, 3};
+    case 56: return {4, 4, 4, 4};
+    case 60: return {6, 6, 6, 6};
+    case 63: return {3, 3, 3, 3};
     case 66: return {22, 22, 22, 22};
     case 70: return {10, 10, 10, 10};
-    case 72: return {6, 9, 6, 9};
+    case 72: return {6, 6, 6, 6};
     case 75: return {5, 5, 5, 5};
     case 78: return {26, 26, 26, 26};
-    case 80: return {10, 10, 10, 10};
-    case 81: return {9, 9, 9, 9};
-    case 84: return {6, 12, 6, 12};
+    case 80: return {4, 4, 4, 4};
+    case 81: return {3, 3, 3, 3};
+    case 84: return {6, 6, 6, 6};
     case 88: return {22, 22, 22, 22};
-    case 90: return {10, 10, 10, 10};
+    case 90: return {6, 6, 6, 6};
     case 98: return {7, 7, 7, 7};
     case 100: return {5, 5, 5, 5};
     case 104: return {26, 26, 26, 26};
     case 105: return {15, 15, 15, 15};
-    case 108: return {6, 9, 6, 9};
+    case 108: return {6, 6, 6, 6};
     case 110: return {22, 22, 22, 22};
-    case 112: return {8, 8, 8, 8};
+    case 112: return {4, 4, 4, 4};
     case 120: return {15, 15, 15, 15};
     case 125: return {5, 5, 5, 5};
     case 126: return {6, 6, 6, 6};
     case 130: return {26, 26, 26, 26};
     case 132: return {22, 22, 22, 22};
     case 135: return {9, 9, 9, 9};
-    case 140: return {14, 14, 14, 14};
+    case 140: return {10, 10, 10, 10};
     case 144: return {9, 9, 9, 9};
     case 147: return {7, 7, 7, 7};
-    case 150: return {5, 10, 5, 10};
+    case 150: return {5, 5, 5, 5};
     case 154: return {22, 22, 22, 22};
     case 156: return {26, 26, 26, 26};
     case 160: return {10, 10, 10, 10};
     case 162: return {9, 9, 9, 9};
-    case 168: return {21, 14, 21, 14};
+    case 168: return {12, 14, 12, 14};
     case 175: return {25, 25, 25, 25};
     case 176: return {22, 22, 22, 22};
     case 180: return {12, 15, 12, 15};
