@@ -196,11 +196,12 @@ constexpr bool schedule_ok(int n, int e) {
 // 5 8 3; 144 = 16 x 9: 3 2 2 2 2 3). Lanes per line not dividing the wave
 // leave spare lanes (50 = 10 x 5, 90 = 9 x 10, ...). Radix 7, 11 and 13 take
 // their roots from the line's twiddle table; lengths with a factor 11 or 13
-// need lines of 22 or 26 elements. Measured on the MI355X, lines of 5 to 14
+// need lines of 22 or 26 elements. Measured on the MI355X, lines of 3 to 10
 // elements run fastest (lines of 21 to 27 held up to 256 VGPRs: 81^3 FP64
-// 13.2k -> 23.5k pairs/s at 9 elements), except where that would take 20 or
-// more lanes per line with spare lanes (120, 240, 400, 432 in FP64; 288, 320
-// in FP32); FP32 448 and 512 take longer lines (28; 16 and, for z, 32) so
+// 13.2k -> 32.4k pairs/s at 3 elements), except at 120, 200 and 240, at 400
+// and 432 in FP64 and at 288 and 320 in FP32, which keep longer lines (their
+// shorter ones measured 3 to 30% slower); FP32 448 and 512 take longer
+// lines (28; 16 and, for z, 32) so
 // that a workgroup's row segments are 64 to 128 bytes (512^3: +36%). Odd
 // lengths run on the same lines; an R2C plane with an odd row count leaves
 // the last packed row pair with one row.
@@ -245,27 +246,27 @@ constexpr LaneElems lane_elems(int n) {
     case 132: return {22, 22, 22, 22};
     case 135: return {9, 9, 9, 9};
     case 140: return {10, 10, 10, 10};
-    case 144: return {9, 9, 9, 9};
+    case 144: return {8, 8, 8, 8};
     case 147: return {7, 7, 7, 7};
     case 150: return {5, 5, 5, 5};
     case 154: return {22, 22, 22, 22};
     case 156: return {26, 26, 26, 26};
-    case 160: return {10, 10, 10, 10};
-    case 162: return {9, 9, 9, 9};
+    case 160: return {8, 8, 8, 8};
+    case 162: return {6, 6, 6, 6};
     case 168: return {12, 14, 12, 14};
     case 175: return {25, 25, 25, 25};
     case 176: return {22, 22, 22, 22};
     case 180: return {12, 15, 12, 15};
     case 182: return {26, 26, 26, 26};
     case 189: return {9, 9, 9, 9};
-    case 196: return {14, 14, 14, 14};
+    case 196: return {7, 7, 7, 7};
     case 198: return {22, 22, 22, 22};
     case 200: return {10, 10, 10, 10};
     case 208: return {26, 26, 26, 26};
     case 210: return {10, 10, 10, 10};
     case 216: return {12, 12, 12, 12};
     case 220: return {22, 22, 22, 22};
-    case 224: return {14, 14, 14, 14};
+    case 224: return {8, 14, 8, 14};
     case 225: return {15, 15, 15, 15};
     case 234: return {26, 26, 26, 26};
     case 240: return {15, 15, 15, 15};
