@@ -224,14 +224,14 @@ constexpr LaneElems lane_elems(int n) {
     case 63: return {3, 3, 3, 3};
     case 66: return {22, 22, 22, 22};
     case 70: return {10, 10, 10, 10};
-    case 72: return {6, 6, 6, 6};
+    case 72: return {4, 4, 4, 4};
     case 75: return {5, 5, 5, 5};
     case 78: return {26, 26, 26, 26};
     case 80: return {4, 4, 4, 4};
     case 81: return {3, 3, 3, 3};
     case 84: return {6, 6, 6, 6};
     case 88: return {22, 22, 22, 22};
-    case 90: return {6, 6, 6, 6};
+    case 90: return {3, 3, 3, 3};
     case 98: return {7, 7, 7, 7};
     case 100: return {5, 5, 5, 5};
     case 104: return {26, 26, 26, 26};
@@ -256,9 +256,9 @@ constexpr LaneElems lane_elems(int n) {
     case 168: return {12, 14, 12, 14};
     case 175: return {25, 25, 25, 25};
     case 176: return {22, 22, 22, 22};
-    case 180: return {12, 15, 12, 15};
+    case 180: return {12, 12, 12, 12};
     case 182: return {26, 26, 26, 26};
-    case 189: return {9, 9, 9, 9};
+    case 189: return {3, 9, 3, 9};
     case 196: return {7, 7, 7, 7};
     case 198: return {22, 22, 22, 22};
     case 200: return {10, 10, 10, 10};
@@ -267,7 +267,7 @@ constexpr LaneElems lane_elems(int n) {
     case 216: return {12, 12, 12, 12};
     case 220: return {22, 22, 22, 22};
     case 224: return {8, 14, 8, 14};
-    case 225: return {15, 15, 15, 15};
+    case 225: return {9, 9, 9, 9};
     case 234: return {26, 26, 26, 26};
     case 240: return {15, 15, 15, 15};
     case 243: return {9, 9, 9, 9};
