@@ -222,11 +222,11 @@ constexpr LaneElems lane_elems(int n) {
     case 56: return {4, 4, 4, 4};
     case 60: return {6, 6, 6, 6};
     case 63: return {3, 3, 3, 3};
-    case 66: return {22, 22, 22, 22};
+    case 66: return {6, 6, 6, 6};
     case 70: return {10, 10, 10, 10};
     case 72: return {4, 4, 4, 4};
     case 75: return {5, 5, 5, 5};
-    case 78: return {26, 26, 26, 26};
+    case 78: return {6, 6, 6, 6};
     case 80: return {4, 4, 4, 4};
     case 81: return {3, 3, 3, 3};
     case 84: return {6, 6, 6, 6};
@@ -237,43 +237,43 @@ constexpr LaneElems lane_elems(int n) {
     case 104: return {26, 26, 26, 26};
     case 105: return {15, 15, 15, 15};
     case 108: return {6, 6, 6, 6};
-    case 110: return {22, 22, 22, 22};
+    case 110: return {10, 10, 10, 10};
     case 112: return {4, 4, 4, 4};
     case 120: return {15, 15, 15, 15};
     case 125: return {5, 5, 5, 5};
     case 126: return {6, 6, 6, 6};
-    case 130: return {26, 26, 26, 26};
-    case 132: return {22, 22, 22, 22};
+    case 130: return {10, 10, 10, 10};
+    case 132: return {12, 12, 12, 12};
     case 135: return {9, 9, 9, 9};
     case 140: return {10, 10, 10, 10};
     case 144: return {8, 8, 8, 8};
     case 147: return {7, 7, 7, 7};
     case 150: return {5, 5, 5, 5};
-    case 154: return {22, 22, 22, 22};
-    case 156: return {26, 26, 26, 26};
+    case 154: return {14, 14, 14, 14};
+    case 156: return {12, 12, 12, 12};
     case 160: return {8, 8, 8, 8};
     case 162: return {6, 6, 6, 6};
     case 168: return {12, 14, 12, 14};
     case 175: return {25, 25, 25, 25};
     case 176: return {22, 22, 22, 22};
     case 180: return {12, 12, 12, 12};
-    case 182: return {26, 26, 26, 26};
+    case 182: return {14, 14, 14, 14};
     case 189: return {3, 9, 3, 9};
     case 196: return {7, 7, 7, 7};
-    case 198: return {22, 22, 22, 22};
+    case 198: return {18, 18, 18, 18};
     case 200: return {10, 10, 10, 10};
     case 208: return {26, 26, 26, 26};
     case 210: return {10, 10, 10, 10};
     case 216: return {12, 12, 12, 12};
-    case 220: return {22, 22, 22, 22};
+    case 220: return {20, 20, 20, 20};
     case 224: return {8, 14, 8, 14};
     case 225: return {9, 9, 9, 9};
-    case 234: return {26, 26, 26, 26};
+    case 234: return {18, 18, 18, 18};
     case 240: return {15, 15, 15, 15};
     case 243: return {9, 9, 9, 9};
     case 252: return {12, 12, 12, 12};
     case 256: return {8, 8, 4, 8};
-    case 260: return {26, 26, 26, 26};
+    case 260: return {20, 20, 20, 20};
     case 264: return {22, 22, 22, 22};
     case 270: return {9, 9, 9, 9};
     case 280: return {14, 14, 14, 14};
