@@ -213,11 +213,11 @@ constexpr LaneElems lane_elems(int n) {
     case 36: return {3, 3, 3, 3};
     case 40: return {4, 4, 4, 4};
     case 42: return {6, 6, 6, 6};
-    case 44: return {22, 22, 22, 22};
+    case 44: return {2, 2, 2, 2};
     case 45: return {3, 3, 3, 3};
     case 48: return {4, 4, 4, 4};
     case 50: return {5, 5, 5, 5};
-    case 52: return {26, 26, 26, 26};
+    case 52: return {2, 2, 2, 2};
     case 54: return {3, 3, 3, 3};
     case 56: return {4, 4, 4, 4};
     case 60: return {6, 6, 6, 6};
@@ -230,11 +230,11 @@ constexpr LaneElems lane_elems(int n) {
     case 80: return {4, 4, 4, 4};
     case 81: return {3, 3, 3, 3};
     case 84: return {6, 6, 6, 6};
-    case 88: return {22, 22, 22, 22};
+    case 88: return {4, 4, 4, 4};
     case 90: return {3, 3, 3, 3};
     case 98: return {7, 7, 7, 7};
     case 100: return {5, 5, 5, 5};
-    case 104: return {26, 26, 26, 26};
+    case 104: return {4, 4, 4, 4};
     case 105: return {15, 15, 15, 15};
     case 108: return {6, 6, 6, 6};
     case 110: return {10, 10, 10, 10};
@@ -255,14 +255,14 @@ constexpr LaneElems lane_elems(int n) {
     case 162: return {6, 6, 6, 6};
     case 168: return {12, 14, 12, 14};
     case 175: return {25, 25, 25, 25};
-    case 176: return {22, 22, 22, 22};
+    case 176: return {16, 16, 16, 16};
     case 180: return {12, 12, 12, 12};
     case 182: return {14, 14, 14, 14};
     case 189: return {3, 9, 3, 9};
     case 196: return {7, 7, 7, 7};
     case 198: return {18, 18, 18, 18};
     case 200: return {10, 10, 10, 10};
-    case 208: return {26, 26, 26, 26};
+    case 208: return {16, 16, 16, 16};
     case 210: return {10, 10, 10, 10};
     case 216: return {12, 12, 12, 12};
     case 220: return {20, 20, 20, 20};
@@ -279,14 +279,14 @@ constexpr LaneElems lane_elems(int n) {
     case 280: return {14, 14, 14, 14};
     case 288: return {9, 18, 9, 18};
     case 300: return {10, 10, 10, 10};
-    case 312: return {26, 26, 26, 26};
+    case 312: return {24, 24, 24, 24};
     case 320: return {10, 20, 10, 10};
     case 330: return {22, 22, 22, 22};
     case 336: return {21, 21, 21, 21};
     case 352: return {22, 22, 22, 22};
     case 360: return {12, 12, 12, 12};
     case 384: return {12, 24, 12, 12};
-    case 390: return {26, 26, 26, 26};
+    case 390: return {15, 15, 15, 15};
     case 400: return {25, 25, 25, 25};
     case 416: return {26, 26, 26, 26};
     case 420: return {15, 15, 15, 15};
@@ -294,7 +294,7 @@ constexpr LaneElems lane_elems(int n) {
     case 440: return {22, 22, 22, 22};
     case 448: return {14, 28, 14, 28};
     case 450: return {15, 15, 15, 15};
-    case 462: return {22, 22, 22, 22};
+    case 462: return {21, 21, 21, 21};
     case 480: return {15, 15, 15, 15};
     case 500: return {10, 10, 10, 10};
     case 512: return {8, 16, 8, 32};
