@@ -27,7 +27,7 @@ namespace spfft {
 extern __shared__ __attribute__((aligned(16))) unsigned char spfft_lds_fast[];
 
 // One call into the z kernels or the split plane stage of one length. Those
-// kernels are compiled in three groups of lengths (three translation units per
+// kernels are compiled in four groups of lengths (four translation units per
 // precision and family, so that they build in parallel); each group's
 // dispatcher launches the call and returns true when it holds the length.
 template <typename T>
@@ -64,6 +64,10 @@ bool z_group2(int n, const ZCall<double>& c);
 bool z_group2(int n, const ZCall<float>& c);
 bool split_group2(int n, const SplitCall<double>& c);
 bool split_group2(int n, const SplitCall<float>& c);
+bool z_group3(int n, const ZCall<double>& c);
+bool z_group3(int n, const ZCall<float>& c);
+bool split_group3(int n, const SplitCall<double>& c);
+bool split_group3(int n, const SplitCall<float>& c);
 
 namespace {
 
@@ -233,13 +237,16 @@ constexpr LaneElems lane_elems(int n) {
     case 88: return {4, 4, 4, 4};
     case 90: return {3, 3, 3, 3};
     case 98: return {7, 7, 7, 7};
+    case 99: return {9, 9, 9, 9};
     case 100: return {5, 5, 5, 5};
     case 104: return {4, 4, 4, 4};
     case 105: return {15, 15, 15, 15};
     case 108: return {6, 6, 6, 6};
     case 110: return {10, 10, 10, 10};
     case 112: return {4, 4, 4, 4};
+    case 117: return {9, 9, 9, 9};
     case 120: return {15, 15, 15, 15};
+    case 121: return {11, 11, 11, 11};
     case 125: return {5, 5, 5, 5};
     case 126: return {6, 6, 6, 6};
     case 130: return {10, 10, 10, 10};
@@ -253,12 +260,15 @@ constexpr LaneElems lane_elems(int n) {
     case 156: return {12, 12, 12, 12};
     case 160: return {8, 8, 8, 8};
     case 162: return {6, 6, 6, 6};
+    case 165: return {15, 15, 15, 15};
     case 168: return {12, 14, 12, 14};
+    case 169: return {13, 13, 13, 13};
     case 175: return {25, 25, 25, 25};
     case 176: return {16, 16, 16, 16};
     case 180: return {12, 12, 12, 12};
     case 182: return {14, 14, 14, 14};
     case 189: return {3, 9, 3, 9};
+    case 195: return {15, 15, 15, 15};
     case 196: return {7, 7, 7, 7};
     case 198: return {18, 18, 18, 18};
     case 200: return {10, 10, 10, 10};
@@ -268,21 +278,30 @@ constexpr LaneElems lane_elems(int n) {
     case 220: return {20, 20, 20, 20};
     case 224: return {8, 14, 8, 14};
     case 225: return {9, 9, 9, 9};
+    case 231: return {21, 21, 21, 21};
     case 234: return {18, 18, 18, 18};
     case 240: return {15, 15, 15, 15};
+    case 242: return {22, 22, 22, 22};
     case 243: return {9, 9, 9, 9};
     case 252: return {12, 12, 12, 12};
     case 256: return {8, 8, 4, 8};
     case 260: return {20, 20, 20, 20};
     case 264: return {22, 22, 22, 22};
     case 270: return {9, 9, 9, 9};
+    case 273: return {21, 21, 21, 21};
+    case 275: return {25, 25, 25, 25};
     case 280: return {14, 14, 14, 14};
+    case 286: return {26, 26, 26, 26};
     case 288: return {9, 18, 9, 18};
+    case 297: return {27, 27, 27, 27};
     case 300: return {10, 10, 10, 10};
     case 312: return {24, 24, 24, 24};
     case 320: return {10, 20, 10, 10};
+    case 325: return {25, 25, 25, 25};
     case 330: return {22, 22, 22, 22};
     case 336: return {21, 21, 21, 21};
+    case 338: return {26, 26, 26, 26};
+    case 351: return {27, 27, 27, 27};
     case 352: return {22, 22, 22, 22};
     case 360: return {12, 12, 12, 12};
     case 384: return {12, 24, 12, 12};
@@ -2295,9 +2314,10 @@ bool fast_length(int n) { return square_length(n) || split_only_length(n); }
     default: throw GPUFFTError();    \
   }
 
-// The fixed lengths, in the three groups of the z kernels and the split plane
-// stage (group 0 holds the class-split lengths; group 2 the lengths with a
-// factor 11 or 13).
+// The fixed lengths, in the four groups of the z kernels and the split plane
+// stage (group 0 holds the class-split lengths; group 2 the even lengths
+// with a factor 11 or 13; group 3 odd multiples of 11 and 13, 11^2 and 13^2
+// and their doubles).
 #define SPFFT_LENGTHS_0(X)                                                              \
   X(32) X(40) X(48) X(50) X(54) X(56) X(60) X(64) X(72) X(80) X(84) X(90) X(96) X(98)  \
   X(100) X(108) X(112) X(120) X(126) X(128) X(144) X(150) X(160) X(162) X(168) X(180) \
@@ -2310,6 +2330,9 @@ bool fast_length(int n) { return square_length(n) || split_only_length(n); }
   X(44) X(52) X(66) X(78) X(88) X(104) X(110) X(130) X(132) X(154) X(156) X(176) X(182)  \
   X(198) X(208) X(220) X(234) X(260) X(264) X(312) X(330) X(352) X(390) X(416) X(440)   \
   X(462)
+#define SPFFT_LENGTHS_3(X)                                                               \
+  X(99) X(117) X(121) X(165) X(169) X(195) X(231) X(242) X(273) X(275) X(286) X(297)   \
+  X(325) X(338) X(351)
 
 }  // namespace
 
@@ -2380,7 +2403,9 @@ void z_call(const ZCall<T>& c) {
 
 template <typename T>
 void z_dispatch(int nz, const ZCall<T>& c) {
-  if (!z_group0(nz, c) && !z_group1(nz, c) && !z_group2(nz, c)) throw GPUFFTError();
+  if (!z_group0(nz, c) && !z_group1(nz, c) && !z_group2(nz, c) && !z_group3(nz, c)) {
+    throw GPUFFTError();
+  }
 }
 
 template <typename T>
@@ -2690,7 +2715,10 @@ void split_call(const SplitCall<T>& c) {
 
 template <typename T>
 void split_dispatch(int n, const SplitCall<T>& c) {
-  if (!split_group0(n, c) && !split_group1(n, c) && !split_group2(n, c)) throw GPUFFTError();
+  if (!split_group0(n, c) && !split_group1(n, c) && !split_group2(n, c) &&
+      !split_group3(n, c)) {
+    throw GPUFFTError();
+  }
 }
 
 template <typename T>
@@ -2774,7 +2802,7 @@ void launch_split_xy_batch(bool forward, bool r2c, int nx, int ny, const FastArg
 // One precision and one kernel family (and, for the z and split kernels, one
 // group of lengths) per translation unit, so that they build in parallel:
 // the fast_kernels_*.hip files include this file with SPFFT_FAST_UNIT set (1
-// to 13); compiled alone it is unit 0.
+// to 17); compiled alone it is unit 0.
 #ifndef SPFFT_FAST_UNIT
 #define SPFFT_FAST_UNIT 0
 #endif
@@ -2808,8 +2836,16 @@ SPFFT_DEFINE_GROUP(z_group2, ZCall, double, SPFFT_Z_CASE, SPFFT_LENGTHS_2)
 SPFFT_DEFINE_GROUP(split_group2, SplitCall, double, SPFFT_SPLIT_CASE, SPFFT_LENGTHS_2)
 #elif SPFFT_FAST_UNIT == 12
 SPFFT_DEFINE_GROUP(z_group2, ZCall, float, SPFFT_Z_CASE, SPFFT_LENGTHS_2)
-#else
+#elif SPFFT_FAST_UNIT == 13
 SPFFT_DEFINE_GROUP(split_group2, SplitCall, float, SPFFT_SPLIT_CASE, SPFFT_LENGTHS_2)
+#elif SPFFT_FAST_UNIT == 14
+SPFFT_DEFINE_GROUP(z_group3, ZCall, double, SPFFT_Z_CASE, SPFFT_LENGTHS_3)
+#elif SPFFT_FAST_UNIT == 15
+SPFFT_DEFINE_GROUP(split_group3, SplitCall, double, SPFFT_SPLIT_CASE, SPFFT_LENGTHS_3)
+#elif SPFFT_FAST_UNIT == 16
+SPFFT_DEFINE_GROUP(z_group3, ZCall, float, SPFFT_Z_CASE, SPFFT_LENGTHS_3)
+#else
+SPFFT_DEFINE_GROUP(split_group3, SplitCall, float, SPFFT_SPLIT_CASE, SPFFT_LENGTHS_3)
 #endif
 
 }  // namespace spfft

@@ -98,12 +98,13 @@ inline int fast_pitch(int nst) {
 
 // True when (nx, ny, nz) have fast kernels: z kernels and the split plane
 // stage for every length in {32, 36, 40, 42, 44, 45, 48, 50, 52, 54, 56, 60,
-// 63, 64, 66, 70, 72, 75, 78, 80, 81, 84, 88, 90, 96, 98, 100, 104, 105, 108,
-// 110, 112, 120, 125, 126, 128, 130, 132, 135, 140, 144, 147, 150, 154, 156,
-// 160, 162, 168, 175, 176, 180, 182, 189, 192, 196, 198, 200, 208, 210, 216,
-// 220, 224, 225, 234, 240, 243, 252, 256, 260, 264, 270, 280, 288, 300, 312,
-// 320, 330, 336, 352, 360, 384, 390, 400, 416, 420, 432, 440, 448, 450, 462,
-// 480, 500, 512}.
+// 63, 64, 66, 70, 72, 75, 78, 80, 81, 84, 88, 90, 96, 98, 99, 100, 104, 105,
+// 108, 110, 112, 117, 120, 121, 125, 126, 128, 130, 132, 135, 140, 144, 147,
+// 150, 154, 156, 160, 162, 165, 168, 169, 175, 176, 180, 182, 189, 192, 195,
+// 196, 198, 200, 208, 210, 216, 220, 224, 225, 231, 234, 240, 242, 243, 252,
+// 256, 260, 264, 270, 273, 275, 280, 286, 288, 297, 300, 312, 320, 325, 330,
+// 336, 338, 351, 352, 360, 384, 390, 400, 416, 420, 432, 440, 448, 450, 462,
+// 480, 500, 512} (108 lengths).
 template <typename T>
 bool fast_path_supported(int nx, int ny, int nz, int ncol, bool r2c);
 // True when the class-split plane kernels serve the plane too: nx = ny in
