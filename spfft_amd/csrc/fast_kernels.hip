@@ -290,7 +290,7 @@ constexpr LaneElems lane_elems(int n) {
     case 270: return {9, 9, 9, 9};
     case 273: return {21, 21, 21, 21};
     case 275: return {25, 25, 25, 25};
-    case 280: return {14, 14, 14, 14};
+    case 280: return {10, 10, 10, 10};
     case 286: return {26, 26, 26, 26};
     case 288: return {9, 18, 9, 18};
     case 297: return {27, 27, 27, 27};
@@ -299,7 +299,7 @@ constexpr LaneElems lane_elems(int n) {
     case 320: return {10, 20, 10, 10};
     case 325: return {25, 25, 25, 25};
     case 330: return {22, 22, 22, 22};
-    case 336: return {21, 21, 21, 21};
+    case 336: return {12, 21, 12, 21};
     case 338: return {26, 26, 26, 26};
     case 351: return {27, 27, 27, 27};
     case 352: return {22, 22, 22, 22};
